@@ -1,0 +1,189 @@
+/*
+ * lumx.h - C ABI of the MI355X-native Lum voxel-lighting hot path.
+ *
+ * Pass-level entry points for the three hot passes of Lum
+ * (radiance probe update, model->world voxelization, glossy reflection march)
+ * plus the world / block-palette / material / model / radiance-buffer state
+ * they run on.  Plain C types only: host pointers are marked _host, device
+ * (HBM) pointers _dev, and `stream` is a hipStream_t passed as void* (NULL =
+ * the context's own stream).
+ *
+ * Every function returns LUMX_OK (0) or a negative LUMX_E* status and never
+ * aborts; lumx_last_error() returns the message of the calling thread's last
+ * failure.  One world may not be used from two threads at once.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   lumx_world_*           LumInternalRenderer::createImages world/radiance/palette/material
+ *                          resources (src/renderer/src/internal_render.cpp:107-209)
+ *                          and Lum::Renderer::setWorldBlock/getWorldBlock
+ *                          (src/renderer/api/renderer.cpp:116-133)
+ *   lumx_world_set_palette updateBlockPalette (src/renderer/src/load_stuff.cpp:36-86)
+ *   lumx_world_set_materials updateMaterialPalette (src/renderer/src/load_stuff.cpp:88-111)
+ *   lumx_model_create      load_mesh / create_RayTrace_VoxelImages
+ *                          (src/renderer/src/load_stuff.cpp:376-385, src/renderer/src/setup.cpp:4-39)
+ *   lumx_model_placement   get_shift + map_mesh push constants + blockify_mesh AABB
+ *                          (src/renderer/src/internal_render.cpp:989-1013,1029-1035,1494-1511)
+ *   lumx_map_models*       start_blockify/blockify_mesh/end_blockify + exec_copies +
+ *                          start_map/map_mesh/end_map -> shaders/map.comp
+ *                          (src/renderer/src/internal_render.cpp:977-1097,1375-1527)
+ *   lumx_radiance_update*  update_radiance -> shaders/radiance.comp
+ *                          (src/renderer/src/internal_render.cpp:1129-1214)
+ *   lumx_select_probes     update_radiance probe selection
+ *                          (src/renderer/src/internal_render.cpp:1142-1168)
+ *   lumx_glossy            glossy_raygen stencil + glossy -> shaders/glossy.frag
+ *                          (src/renderer/src/internal_render.cpp:1990-1998,2027-2037)
+ *   lumx_camera_default    Camera::updateCamera + start_raygen UBO fill
+ *                          (src/renderer/src/internal_render.cpp:923-934,1580-1592)
+ */
+#ifndef LUMX_H
+#define LUMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUMX_OK 0
+#define LUMX_EINVAL (-1)     /* bad argument / handle / size */
+#define LUMX_ENOMEM (-2)     /* device allocation failed or HBM budget exceeded */
+#define LUMX_EDEVICE (-3)    /* HIP runtime error */
+#define LUMX_EOVERFLOW (-4)  /* block palette slots exhausted */
+#define LUMX_EBUSY (-5)      /* serving queue full / budget refused */
+
+#define LUMX_RADIANCE_F32 0  /* RGBA float32 texels, 16 B */
+#define LUMX_RADIANCE_F16 1  /* RGBA float16 texels, 8 B, round to nearest even */
+#define LUMX_RADIANCE_BF16 2 /* RGBA bfloat16 texels, 8 B, round to nearest even */
+
+#define LUMX_PALETTE_XYZ_XFAST 0  /* block voxels indexed x + 16y + 256z */
+#define LUMX_PALETTE_REF_ZFAST 1  /* Lum BlockVoxels[x][y][z] (z fastest) */
+
+typedef struct lumx_context* lumx_context_t;
+typedef struct lumx_world* lumx_world_t;
+typedef struct lumx_model* lumx_model_t;
+
+typedef struct {
+    int size[3];          /* world size in 16^3 blocks (x, y, z) */
+    int static_blocks;    /* static block palette size (slots 0..static-1) */
+    int palette_slots;    /* total palette slots; 0 = world blocks + static (capped at 32768) */
+    int radiance_format;  /* LUMX_RADIANCE_* */
+} lumx_world_desc;
+
+/* Radiance update parameters (radiance.comp push constants + constants). */
+typedef struct {
+    int rays;          /* rays per probe: power of two <= 64 or multiple of 64 <= 1024 */
+    int time_seed;     /* pco.timeSeed (frame index) */
+    int size;          /* pco.size: request stride (>= 1) */
+    int shift;         /* pco.sihft: request offset (0 <= shift < size) */
+    float alpha;       /* REACTIVNESS temporal blend weight */
+    float max_dist;    /* march distance cap in voxels (reference: 128) */
+    int max_steps;     /* coarse-step cap, 0 = none (reference) */
+    int reserved;
+} lumx_radiance_params;
+
+/* Mesh transform, identical layout to LumMeshTransform: rot = (w, x, y, z). */
+typedef struct {
+    float rot[4];
+    float shift[3];
+} lumx_transform;
+
+/* Everything the map pass and blockify consume for one placed model. */
+typedef struct {
+    float minv[16];     /* inverse model transform, column-major (GLSL mat4) */
+    int shift[4];       /* border.min (dispatch origin), w unused */
+    int groups[3];      /* dispatch size in 4x4x4 workgroups */
+    float aabb_min[3];  /* float AABB of get_shift() (used by blockify) */
+    float aabb_max[3];
+} lumx_placement;
+
+/* The glossy pass UBO fields (common/ubo.glsl) it reads. */
+typedef struct {
+    float campos[3];
+    float camdir[3];
+    float horiz_scaled[3];  /* horizline * viewSize.x / 2 */
+    float verti_scaled[3];  /* vertiline * viewSize.y / 2 */
+    float light_dir[3];     /* globalLightDir */
+    float frame_size[2];    /* framebuffer size in pixels */
+} lumx_camera;
+
+/* ---- context / errors ---- */
+int lumx_context_create(int device, lumx_context_t* out);
+int lumx_context_destroy(lumx_context_t ctx);
+int lumx_context_sync(lumx_context_t ctx);
+void* lumx_context_stream(lumx_context_t ctx);
+const char* lumx_last_error(void);
+const char* lumx_version(void);
+
+/* ---- world state ---- */
+int lumx_world_create(lumx_context_t ctx, const lumx_world_desc* desc, lumx_world_t* out);
+int lumx_world_destroy(lumx_world_t w);
+int lumx_world_info(lumx_world_t w, lumx_world_desc* out);
+/* Origin world (x-fastest int16, 0 = air), copied at call time. */
+int lumx_world_set_blocks(lumx_world_t w, const int16_t* blocks_host);
+int lumx_world_get_blocks(lumx_world_t w, int16_t* blocks_host, int blockified);
+int lumx_world_set_block(lumx_world_t w, int x, int y, int z, int16_t block);
+int lumx_world_get_block(lumx_world_t w, int x, int y, int z, int16_t* out);
+/* Static palette blocks [first, first+count): count*4096 voxels. */
+int lumx_world_set_palette(lumx_world_t w, int first, int count, const uint8_t* voxels_host, int layout);
+/* Device palette readback: slots [first, first+count) x-fastest. */
+int lumx_world_get_palette(lumx_world_t w, int first, int count, uint8_t* voxels_host);
+/* 256 materials x {r, g, b, a, emission, roughness}. */
+int lumx_world_set_materials(lumx_world_t w, const float* materials_host);
+/* Radiance field, RGBA float per probe (x-fastest); converted to the storage format. */
+int lumx_world_write_radiance(lumx_world_t w, const float* rgba_host);
+int lumx_world_read_radiance(lumx_world_t w, float* rgba_host);
+/* Device pointer and byte size of the current radiance field. */
+int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes);
+/* Upload the origin world as the current world (no models mapped). */
+int lumx_world_upload_blocks(lumx_world_t w, void* stream);
+
+/* ---- models ---- */
+int lumx_model_create(lumx_context_t ctx, const uint8_t* voxels_host, int sx, int sy, int sz,
+                      lumx_model_t* out);
+int lumx_model_destroy(lumx_model_t m);
+int lumx_model_size(lumx_model_t m, int size[3]);
+/* Host math of map_mesh/get_shift for one transform and model size. */
+int lumx_model_placement(const lumx_transform* t, const int size[3], lumx_placement* out);
+
+/* ---- passes ---- */
+/* blockify (host) + world upload + exec_copies + map.comp for n models in
+ * order (later models win on overlap). */
+int lumx_map_models(lumx_world_t w, int n, const lumx_model_t* models, const lumx_transform* transforms,
+                    void* stream);
+int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models,
+                           const lumx_placement* placements, void* stream);
+/* Palette slots allocated by the last map (the reference's paletteCounter). */
+int lumx_world_palette_counter(lumx_world_t w, int* out);
+
+/* radiance.comp: requests_dev = int32x4 (x, y, z, _) probe list in HBM, or
+ * NULL for every probe in x-fastest order (n_requests ignored).  Reads the
+ * current field, writes a new one (Jacobi), then makes it current. */
+int lumx_radiance_update(lumx_world_t w, const int32_t* requests_dev, int n_requests,
+                         const lumx_radiance_params* p, void* stream);
+/* Same, also writing per-ray hit keys (0 = miss) and {coarse, refine} sample
+ * counts for every dispatched ray (either may be NULL). */
+int lumx_radiance_update_traced(lumx_world_t w, const int32_t* requests_dev, int n_requests,
+                                const lumx_radiance_params* p, int32_t* ray_keys_dev,
+                                int32_t* ray_steps_dev, void* stream);
+/* Probe selection of update_radiance on the current (blockified) world. */
+int lumx_select_probes(lumx_world_t w, const int32_t* special_host, int n_special,
+                       int32_t* requests_host, int* n_out);
+
+/* glossy.frag over a width x height G-buffer (matnorm u8x4, depth = view
+ * depth / 1000).  Pixels whose material roughness > 0.5 are not written. */
+int lumx_glossy(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_dev, int width,
+                int height, const lumx_camera* cam, float max_dist, int max_steps, float* out_dev,
+                int32_t* hit_keys_dev, void* stream);
+/* Lum's default camera (internal_types.hpp:54-66) for a frame size. */
+int lumx_camera_default(int frame_w, int frame_h, lumx_camera* out);
+
+/* ---- timing ---- */
+/* Time (ms) of the last pass of each kind on this world (0 radiance, 1 map, 2 glossy),
+ * measured with HIP events on the pass's stream; valid after a sync. */
+int lumx_world_last_pass_ms(lumx_world_t w, int pass, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUMX_H */

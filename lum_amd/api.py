@@ -1,0 +1,199 @@
+"""Python view of the lumx C ABI (include/lumx.h).
+
+Device buffers are torch tensors on the HIP device (torch is used for HBM
+allocation and streams only); every pass runs in liblumx's HIP kernels on
+torch's current stream.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (Camera, Placement, RadianceParams, Transform, WorldDesc, check, load, RADIANCE_F32,
+                   PALETTE_XYZ_XFAST)
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.lib = load()
+        self.handle = C.c_void_p()
+        check(self.lib.lumx_context_create(device, C.byref(self.handle)))
+        self.device = device
+
+    def sync(self):
+        check(self.lib.lumx_context_sync(self.handle))
+
+    def close(self):
+        if self.handle:
+            check(self.lib.lumx_context_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+
+class Model:
+    def __init__(self, ctx, voxels):
+        """voxels: uint8 array indexed [z][y][x] (x fastest in memory)."""
+        v = np.ascontiguousarray(voxels, dtype=np.uint8)
+        self.lib = ctx.lib
+        self.size = (v.shape[2], v.shape[1], v.shape[0])
+        self.handle = C.c_void_p()
+        check(self.lib.lumx_model_create(ctx.handle, _np_ptr(v), self.size[0], self.size[1], self.size[2],
+                                         C.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            check(self.lib.lumx_model_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+
+def placement(rot_wxyz, shift, size):
+    """Host math of map_mesh for one model: returns a Placement."""
+    lib = load()
+    t = Transform((C.c_float * 4)(*rot_wxyz), (C.c_float * 3)(*shift))
+    sz = (C.c_int * 3)(*size)
+    p = Placement()
+    check(lib.lumx_model_placement(C.byref(t), sz, C.byref(p)))
+    return p
+
+
+def camera_default(width, height):
+    cam = Camera()
+    check(load().lumx_camera_default(width, height, C.byref(cam)))
+    return cam
+
+
+def radiance_params(rays=64, time_seed=0, size=1, shift=0, alpha=0.1, max_dist=128.0, max_steps=0):
+    return RadianceParams(rays, time_seed, size, shift, alpha, max_dist, max_steps, 0)
+
+
+class World:
+    def __init__(self, ctx, size, static_blocks, palette_slots=0, radiance_format=RADIANCE_F32):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.size = tuple(int(s) for s in size)
+        d = WorldDesc((C.c_int * 3)(*self.size), static_blocks, palette_slots, radiance_format)
+        self.handle = C.c_void_p()
+        check(self.lib.lumx_world_create(ctx.handle, C.byref(d), C.byref(self.handle)))
+        info = WorldDesc()
+        check(self.lib.lumx_world_info(self.handle, C.byref(info)))
+        self.static_blocks = info.static_blocks
+        self.palette_slots = info.palette_slots
+        self.radiance_format = radiance_format
+
+    @property
+    def n_blocks(self):
+        return self.size[0] * self.size[1] * self.size[2]
+
+    def close(self):
+        if self.handle:
+            check(self.lib.lumx_world_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    # -- state --
+    def set_blocks(self, blocks):
+        b = np.ascontiguousarray(blocks, dtype=np.int16)
+        assert b.size == self.n_blocks
+        check(self.lib.lumx_world_set_blocks(self.handle, _np_ptr(b)))
+
+    def get_blocks(self, blockified=True):
+        out = np.zeros((self.size[2], self.size[1], self.size[0]), np.int16)
+        check(self.lib.lumx_world_get_blocks(self.handle, _np_ptr(out), 1 if blockified else 0))
+        return out
+
+    def set_palette(self, voxels, first=0, layout=PALETTE_XYZ_XFAST):
+        v = np.ascontiguousarray(voxels, dtype=np.uint8)
+        count = v.size // 4096
+        check(self.lib.lumx_world_set_palette(self.handle, first, count, _np_ptr(v), layout))
+
+    def get_palette(self, first, count):
+        out = np.zeros((count, 16, 16, 16), np.uint8)
+        check(self.lib.lumx_world_get_palette(self.handle, first, count, _np_ptr(out)))
+        return out
+
+    def set_materials(self, mats):
+        m = np.ascontiguousarray(mats, dtype=np.float32)
+        assert m.size == 256 * 6
+        check(self.lib.lumx_world_set_materials(self.handle, _np_ptr(m)))
+
+    def write_radiance(self, rgba):
+        r = np.ascontiguousarray(rgba, dtype=np.float32)
+        assert r.size == self.n_blocks * 4
+        check(self.lib.lumx_world_write_radiance(self.handle, _np_ptr(r)))
+
+    def read_radiance(self):
+        out = np.zeros((self.size[2], self.size[1], self.size[0], 4), np.float32)
+        check(self.lib.lumx_world_read_radiance(self.handle, _np_ptr(out)))
+        return out
+
+    def upload_blocks(self):
+        check(self.lib.lumx_world_upload_blocks(self.handle, _stream()))
+
+    def palette_counter(self):
+        n = C.c_int()
+        check(self.lib.lumx_world_palette_counter(self.handle, C.byref(n)))
+        return n.value
+
+    def last_pass_ms(self, kind):
+        ms = C.c_float()
+        check(self.lib.lumx_world_last_pass_ms(self.handle, kind, C.byref(ms)))
+        return ms.value
+
+    # -- passes --
+    def map_models(self, models, rots, shifts):
+        n = len(models)
+        hs = (C.c_void_p * max(n, 1))(*[m.handle.value for m in models])
+        ts = (Transform * max(n, 1))(*[Transform((C.c_float * 4)(*r), (C.c_float * 3)(*s))
+                                       for r, s in zip(rots, shifts)])
+        check(self.lib.lumx_map_models(self.handle, n, hs, ts, _stream()))
+
+    def map_models_placed(self, models, placements):
+        n = len(models)
+        hs = (C.c_void_p * max(n, 1))(*[m.handle.value for m in models])
+        ps = (Placement * max(n, 1))(*placements)
+        check(self.lib.lumx_map_models_placed(self.handle, n, hs, ps, _stream()))
+
+    def radiance_update(self, requests=None, params=None, traced=False, **kw):
+        """requests: torch int32 (n, 4) on the device, or None for every probe."""
+        import torch
+        p = params if params is not None else radiance_params(**kw)
+        n = 0 if requests is None else int(requests.shape[0])
+        if not traced:
+            check(self.lib.lumx_radiance_update(self.handle, _ptr(requests), n, C.byref(p), _stream()))
+            return None
+        n_total = n if requests is not None else self.n_blocks
+        n_rays = (n_total // p.size) * p.rays
+        keys = torch.zeros(max(n_rays, 1), dtype=torch.int32, device="cuda")
+        steps = torch.zeros(max(2 * n_rays, 2), dtype=torch.int32, device="cuda")
+        check(self.lib.lumx_radiance_update_traced(self.handle, _ptr(requests), n, C.byref(p), _ptr(keys),
+                                                   _ptr(steps), _stream()))
+        return keys[:n_rays], steps[:2 * n_rays].view(-1, 2)
+
+    def select_probes(self, special=None):
+        sp = np.zeros((0, 4), np.int32) if special is None else np.ascontiguousarray(special, np.int32)
+        out = np.zeros((self.n_blocks + len(sp), 4), np.int32)
+        n = C.c_int()
+        check(self.lib.lumx_select_probes(self.handle, _np_ptr(sp), len(sp), _np_ptr(out), C.byref(n)))
+        return out[: n.value]
+
+    def glossy(self, matnorm, depth, cam, max_dist=128.0, max_steps=0, out=None, traced=False):
+        """matnorm: torch uint8 (h, w, 4); depth: torch float32 (h, w); returns float32 (h, w, 4)."""
+        import torch
+        h, w = int(depth.shape[0]), int(depth.shape[1])
+        if out is None:
+            out = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+        keys = torch.zeros((h, w), dtype=torch.int32, device="cuda") if traced else None
+        check(self.lib.lumx_glossy(self.handle, _ptr(matnorm), _ptr(depth), w, h, C.byref(cam),
+                                   float(max_dist), int(max_steps), _ptr(out), _ptr(keys), _stream()))
+        return (out, keys) if traced else out

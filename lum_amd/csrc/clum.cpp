@@ -1,0 +1,488 @@
+// clum.cpp - the headless lum_* C API of include/clum.h on top of lumx.h.
+//
+// Mirrors the hot-path subset of the reference's C99 API
+// (src/renderer/api/crenderer.cpp:12-209) and the frame sequencing of
+// Lum::Renderer (src/renderer/api/renderer.cpp:52-549): host-side origin
+// world and block palette, per-frame model queue with screen-space culling
+// (is_model_visible, renderer.cpp:220-257) and camera-distance ordering
+// (prepareFrame, renderer.cpp:401-420), then on submit the GPU passes
+// map -> radiance -> glossy.  The radiance work-size controller of
+// update_radiance (internal_render.cpp:1190-1201) is kept, driven by the
+// measured radiance pass time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/clum.h"
+#include "../../include/lumx.h"
+
+namespace {
+
+struct DrawRequest {
+    lumx_model_t model;
+    LumMeshTransform trans;
+    double cam_dist;
+};
+
+struct Renderer {
+    LumSettings settings{};
+    lumx_context_t ctx = nullptr;
+    lumx_world_t world = nullptr;
+    std::vector<std::array<uint8_t, 4096>> blocks;  // BlockVoxels[x][y][z] per static block
+    std::vector<float> mats = std::vector<float>(256 * 6, 0.0f);
+    std::vector<DrawRequest> mesh_que;
+    std::vector<int32_t> special;  // specialRadianceUpdates, int32x4
+    std::vector<lumx_model_t> models;
+    double cam[4][4];              // cameraTransform (identity until the first frame)
+    int frame = 0;
+    int magic_size = 2;
+    float radiance_ms = 0.0f;
+    int32_t* req_dev = nullptr;
+    std::vector<int32_t> req_host;
+    const uint8_t* gb_matnorm = nullptr;
+    const float* gb_depth = nullptr;
+    float* gb_out = nullptr;
+    int gb_w = 0, gb_h = 0;
+    double ts[4] = {0, 0, 0, 0};
+    bool should_close = false;
+};
+
+std::mutex g_live_mu;
+std::set<Renderer*> g_live;
+
+Renderer* get(LumRenderer p) {
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    Renderer* r = (Renderer*)p;
+    return g_live.count(r) ? r : nullptr;
+}
+
+void ident(double m[4][4]) {
+    for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < 4; ++k) m[c][k] = (c == k) ? 1.0 : 0.0;
+}
+
+void matmul(const double a[4][4], const double b[4][4], double o[4][4]) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < 4; ++k) s += a[k][r] * b[c][k];
+            o[c][r] = s;
+        }
+}
+
+double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+void cross3(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - b[1] * a[2];
+    o[1] = a[2] * b[0] - b[2] * a[0];
+    o[2] = a[0] * b[1] - b[0] * a[1];
+}
+void norm3(double v[3]) {
+    double l = std::sqrt(dot3(v, v));
+    v[0] /= l; v[1] /= l; v[2] /= l;
+}
+
+// Camera::updateCamera (internal_render.cpp:923-934) with the default camera:
+// worldToScreen = ortho(-w/2, w/2, h/2, -h/2, 0, 2000) * lookAt(pos, pos+dir, +z),
+// right-handed, clip depth in [-1, 1].
+void update_camera(double out[4][4]) {
+    float fd[3] = {0.61f, 1.0f, -0.8f};
+    float l = std::sqrt(fd[0] * fd[0] + fd[1] * fd[1] + fd[2] * fd[2]);
+    double dir[3] = {(double)(fd[0] / l), (double)(fd[1] / l), (double)(fd[2] / l)};
+    double eye[3] = {60.0, 0.0, 194.0};
+    double up[3] = {0.0, 0.0, 1.0};
+    double f[3] = {dir[0], dir[1], dir[2]};
+    norm3(f);
+    double s[3];
+    cross3(f, up, s);
+    norm3(s);
+    double u[3];
+    cross3(s, f, u);
+    double view[4][4];
+    ident(view);
+    view[0][0] = s[0]; view[1][0] = s[1]; view[2][0] = s[2];
+    view[0][1] = u[0]; view[1][1] = u[1]; view[2][1] = u[2];
+    view[0][2] = -f[0]; view[1][2] = -f[1]; view[2][2] = -f[2];
+    view[3][0] = -dot3(s, eye);
+    view[3][1] = -dot3(u, eye);
+    view[3][2] = dot3(f, eye);
+    const double vw = 1920.0 / 5.0, vh = 1080.0 / 5.0;
+    const double L = -vw / 2.0, R = vw / 2.0, B = vh / 2.0, T = -vh / 2.0, N = -0.0, F = 2000.0;
+    double proj[4][4];
+    ident(proj);
+    proj[0][0] = 2.0 / (R - L);
+    proj[1][1] = 2.0 / (T - B);
+    proj[2][2] = -2.0 / (F - N);
+    proj[3][0] = -(R + L) / (R - L);
+    proj[3][1] = -(T + B) / (T - B);
+    proj[3][2] = -(F + N) / (F - N);
+    matmul(proj, view, out);
+}
+
+// Rotate v by unit quaternion q = (w, x, y, z) (FP32).
+void quat_rotate(const float q[4], const float v[3], float o[3]) {
+    const float qv[3] = {q[1], q[2], q[3]};
+    float uv[3] = {qv[1] * v[2] - v[1] * qv[2], qv[2] * v[0] - v[2] * qv[0], qv[0] * v[1] - v[0] * qv[1]};
+    float uuv[3] = {qv[1] * uv[2] - uv[1] * qv[2], qv[2] * uv[0] - uv[2] * qv[0], qv[0] * uv[1] - uv[0] * qv[1]};
+    for (int k = 0; k < 3; ++k) o[k] = v[k] + ((uv[k] * q[0]) + uuv[k]) * 2.0f;
+}
+
+// is_model_visible (renderer.cpp:220-257): any transformed corner of the
+// model box inside the clip cube of the FP32 view-projection.
+bool model_visible(const int size[3], const LumMeshTransform& t, const double cam[4][4]) {
+    float m[4][4];
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) m[c][r] = (float)cam[c][r];
+    for (int x = 0; x <= 1; ++x)
+        for (int y = 0; y <= 1; ++y)
+            for (int z = 0; z <= 1; ++z) {
+                float corner[3] = {(float)x * (float)size[0], (float)y * (float)size[1], (float)z * (float)size[2]};
+                float p[3];
+                quat_rotate(t.rot, corner, p);
+                for (int k = 0; k < 3; ++k) p[k] += t.shift[k];
+                float clip[4];
+                for (int r = 0; r < 4; ++r) clip[r] = m[0][r] * p[0] + m[1][r] * p[1] + m[2][r] * p[2] + m[3][r];
+                if (clip[3] != 0.0f)
+                    for (int r = 0; r < 4; ++r) clip[r] /= clip[3];
+                if (clip[0] >= -1.0f && clip[1] >= -1.0f && clip[2] >= -1.0f && clip[0] <= 1.0f &&
+                    clip[1] <= 1.0f && clip[2] <= 1.0f)
+                    return true;
+            }
+    return false;
+}
+
+void load_scene_file(Renderer* r, const char* path) {
+    if (!r->world || !path) return;
+    FILE* f = std::fopen(path, "rb");
+    const int wx = r->settings.world_size[0], wy = r->settings.world_size[1], wz = r->settings.world_size[2];
+    std::vector<int16_t> world((size_t)wx * wy * wz, 0);
+    if (f) {
+        std::vector<char> buf;
+        char tmp[65536];
+        size_t n;
+        while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+        std::fclose(f);
+        // 12-byte ivec3 header, then int16 x-fastest blocks, clamped to
+        // [0, static] (load_stuff.cpp:187-217).
+        if (buf.size() >= 12) {
+            int hs[3];
+            std::memcpy(hs, buf.data(), 12);
+            const int16_t* stored = (const int16_t*)(buf.data() + 12);
+            size_t n_stored = (buf.size() - 12) / 2;
+            int rx = std::min(std::max(hs[0], 0), wx), ry = std::min(std::max(hs[1], 0), wy),
+                rz = std::min(std::max(hs[2], 0), wz);
+            for (int z = 0; z < rz; ++z)
+                for (int y = 0; y < ry; ++y)
+                    for (int x = 0; x < rx; ++x) {
+                        size_t idx = (size_t)x + (size_t)hs[0] * y + (size_t)hs[0] * hs[1] * z;
+                        if (idx >= n_stored) continue;
+                        int16_t b = stored[idx];
+                        b = std::min<int16_t>(std::max<int16_t>(b, 0), (int16_t)r->settings.static_block_palette_size);
+                        world[x + (size_t)wx * (y + (size_t)wy * z)] = b;
+                    }
+        }
+    }
+    lumx_world_set_blocks(r->world, world.data());
+}
+
+}  // namespace
+
+extern "C" {
+
+LumRenderer lum_create_instance(const LumSettings* settings) {
+    Renderer* r = new Renderer();
+    if (settings) r->settings = *settings;
+    ident(r->cam);
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    g_live.insert(r);
+    return r;
+}
+
+static void release(Renderer* r) {
+    if (r->ctx) lumx_context_sync(r->ctx);
+    for (lumx_model_t m : r->models) lumx_model_destroy(m);
+    r->models.clear();
+    if (r->world) lumx_world_destroy(r->world);
+    r->world = nullptr;
+    if (r->req_dev) hipFree(r->req_dev);
+    r->req_dev = nullptr;
+    if (r->ctx) lumx_context_destroy(r->ctx);
+    r->ctx = nullptr;
+}
+
+void lum_destroy_instance(LumRenderer instance) {
+    Renderer* r = get(instance);
+    if (!r) return;
+    release(r);
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        g_live.erase(r);
+    }
+    delete r;
+}
+
+void lum_demo_load_scene(LumRenderer instance, const char* path) {
+    Renderer* r = get(instance);
+    if (r) load_scene_file(r, path);
+}
+
+void lum_init(LumRenderer instance, const LumSettings* settings) {
+    Renderer* r = get(instance);
+    if (!r || !settings) return;
+    r->settings = *settings;
+    if (lumx_context_create(0, &r->ctx) != LUMX_OK) return;
+    lumx_world_desc d{};
+    for (int a = 0; a < 3; ++a) d.size[a] = settings->world_size[a];
+    d.static_blocks = settings->static_block_palette_size;
+    d.palette_slots = 0;
+    d.radiance_format = LUMX_RADIANCE_F32;
+    if (lumx_world_create(r->ctx, &d, &r->world) != LUMX_OK) return;
+    r->blocks.assign(settings->static_block_palette_size, std::array<uint8_t, 4096>{});
+    size_t nb = (size_t)d.size[0] * d.size[1] * d.size[2];
+    if (hipMalloc(&r->req_dev, nb * 16) != hipSuccess) r->req_dev = nullptr;
+    r->req_host.resize(nb * 4);
+}
+
+void lum_load_world(LumRenderer instance, const LumMeshBlock* world_data) {
+    Renderer* r = get(instance);
+    if (r && r->world && world_data) lumx_world_set_blocks(r->world, world_data);
+}
+
+void lum_load_world_file(LumRenderer instance, const char* file) {
+    Renderer* r = get(instance);
+    if (r) load_scene_file(r, file);
+}
+
+void lum_set_world_block(LumRenderer instance, int x, int y, int z, LumMeshBlock block) {
+    Renderer* r = get(instance);
+    if (r && r->world) lumx_world_set_block(r->world, x, y, z, block);
+}
+
+LumMeshBlock lum_get_world_block(const LumRenderer instance, int x, int y, int z) {
+    Renderer* r = get(instance);
+    int16_t b = 0;
+    if (r && r->world) lumx_world_get_block(r->world, x, y, z, &b);
+    return b;
+}
+
+void lum_load_block(LumRenderer instance, LumMeshBlock block, void* block_data) {
+    Renderer* r = get(instance);
+    if (!r || !block_data || block < 0 || block >= (int)r->blocks.size()) return;
+    std::memcpy(r->blocks[block].data(), block_data, 4096);
+}
+
+void lum_load_block_file(LumRenderer, LumMeshBlock, const char*) {
+    // .vox import is not part of this library (MagicaVoxel reader absent).
+}
+
+void lum_upload_block_palette_to_gpu(LumRenderer instance) {
+    Renderer* r = get(instance);
+    if (!r || !r->world || r->blocks.empty()) return;
+    std::vector<uint8_t> flat(r->blocks.size() * 4096);
+    for (size_t b = 0; b < r->blocks.size(); ++b) std::memcpy(flat.data() + b * 4096, r->blocks[b].data(), 4096);
+    lumx_world_set_palette(r->world, 0, (int)r->blocks.size(), flat.data(), LUMX_PALETTE_REF_ZFAST);
+}
+
+void lum_upload_material_palette_to_gpu(LumRenderer instance) {
+    Renderer* r = get(instance);
+    if (r && r->world) lumx_world_set_materials(r->world, r->mats.data());
+}
+
+LumMeshModel lum_load_mesh_file(LumRenderer, const char*, bool) { return nullptr; }
+
+LumMeshModel lum_load_mesh_data(LumRenderer instance, void* mesh_data, int x_size, int y_size, int z_size) {
+    Renderer* r = get(instance);
+    if (!r || !r->ctx) return nullptr;
+    lumx_model_t m = nullptr;
+    if (lumx_model_create(r->ctx, (const uint8_t*)mesh_data, x_size, y_size, z_size, &m) != LUMX_OK) return nullptr;
+    r->models.push_back(m);
+    return m;
+}
+
+void lum_free_mesh(LumRenderer instance, LumMeshModel model) {
+    Renderer* r = get(instance);
+    if (!r || !model) return;
+    auto it = std::find(r->models.begin(), r->models.end(), (lumx_model_t)model);
+    if (it == r->models.end()) return;
+    lumx_model_destroy(*it);
+    r->models.erase(it);
+}
+
+LumMeshFoliage lum_load_foliage(LumRenderer, const char*, int, int) { return nullptr; }
+LumMeshVolumetric lum_load_volumetric(LumRenderer, float, float, const uint8_t*) { return nullptr; }
+LumMeshLiquid lum_load_liquid(LumRenderer, int, int) { return nullptr; }
+
+void lum_start_frame(LumRenderer instance) {
+    Renderer* r = get(instance);
+    if (r) r->mesh_que.clear();
+}
+
+void lum_draw_world(LumRenderer) {}
+void lum_draw_particles(LumRenderer) {}
+
+void lum_draw_model(LumRenderer instance, const LumMeshModel mesh, const LumMeshTransform* trans) {
+    Renderer* r = get(instance);
+    if (!r || !mesh || !trans) return;
+    int size[3];
+    if (lumx_model_size((lumx_model_t)mesh, size) != LUMX_OK) return;
+    if (!model_visible(size, *trans, r->cam)) return;
+    r->mesh_que.push_back(DrawRequest{(lumx_model_t)mesh, *trans, 0.0});
+}
+
+void lum_draw_foliage_block(LumRenderer, const LumMeshFoliage, const int*) {}
+void lum_draw_liquid_block(LumRenderer, const LumMeshLiquid, const int*) {}
+void lum_draw_volumetric_block(LumRenderer, const LumMeshVolumetric, const int*) {}
+
+void lum_prepare_frame(LumRenderer instance) {
+    Renderer* r = get(instance);
+    if (!r) return;
+    for (DrawRequest& d : r->mesh_que) {
+        double z = 0.0;
+        for (int k = 0; k < 3; ++k) z += r->cam[k][2] * (double)d.trans.shift[k];
+        z += r->cam[3][2];
+        d.cam_dist = -z;
+    }
+    std::stable_sort(r->mesh_que.begin(), r->mesh_que.end(),
+                     [](const DrawRequest& a, const DrawRequest& b) { return a.cam_dist > b.cam_dist; });
+}
+
+void lum_submit_frame(LumRenderer instance) {
+    Renderer* r = get(instance);
+    if (!r || !r->world || !r->req_dev) return;
+    update_camera(r->cam);
+    void* stream = lumx_context_stream(r->ctx);
+    // blockify + exec_copies + map.comp
+    std::vector<lumx_model_t> models;
+    std::vector<lumx_transform> trans;
+    for (const DrawRequest& d : r->mesh_que) {
+        models.push_back(d.model);
+        lumx_transform t;
+        std::memcpy(t.rot, d.trans.rot, sizeof(t.rot));
+        std::memcpy(t.shift, d.trans.shift, sizeof(t.shift));
+        trans.push_back(t);
+    }
+    if (lumx_map_models(r->world, (int)models.size(), models.data(), trans.data(), stream) != LUMX_OK) return;
+    float map_ms = 0.0f;
+    lumx_world_last_pass_ms(r->world, 1, &map_ms);
+    // update_radiance: probe selection, work-size controller, dispatch.
+    int n = 0;
+    if (lumx_select_probes(r->world, r->special.data(), (int)(r->special.size() / 4), r->req_host.data(), &n) !=
+        LUMX_OK)
+        return;
+    r->special.clear();
+    if (r->radiance_ms < 1.8f) r->magic_size--;
+    else if (r->radiance_ms > 2.2f) r->magic_size++;
+    r->magic_size = std::min(std::max(r->magic_size, 1), 10);
+    lumx_radiance_params p{};
+    p.rays = 64;
+    p.time_seed = r->frame;
+    p.size = r->magic_size;
+    p.shift = r->frame % r->magic_size;
+    p.alpha = 0.01f;  // REACTIVNESS default (radiance.comp:28)
+    p.max_dist = 128.0f;
+    p.max_steps = 0;
+    if (n > 0) {
+        if (hipMemcpyAsync(r->req_dev, r->req_host.data(), (size_t)n * 16, hipMemcpyHostToDevice,
+                           (hipStream_t)stream) != hipSuccess)
+            return;
+        if (lumx_radiance_update(r->world, r->req_dev, n, &p, stream) != LUMX_OK) return;
+        lumx_world_last_pass_ms(r->world, 0, &r->radiance_ms);
+    }
+    float glossy_ms = 0.0f;
+    if (r->gb_matnorm && r->gb_depth && r->gb_out) {
+        lumx_camera cam;
+        lumx_camera_default(r->gb_w, r->gb_h, &cam);
+        if (lumx_glossy(r->world, r->gb_matnorm, r->gb_depth, r->gb_w, r->gb_h, &cam, 128.0f, 0, r->gb_out,
+                        nullptr, stream) == LUMX_OK)
+            lumx_world_last_pass_ms(r->world, 2, &glossy_ms);
+    }
+    hipStreamSynchronize((hipStream_t)stream);
+    r->ts[0] = 0.0;
+    r->ts[1] = map_ms;
+    r->ts[2] = r->ts[1] + (n > 0 ? r->radiance_ms : 0.0f);
+    r->ts[3] = r->ts[2] + glossy_ms;
+    r->frame++;
+}
+
+void lum_cleanup(LumRenderer instance) {
+    // The reference's lum_cleanup also deletes the instance (crenderer.cpp:168-171).
+    lum_destroy_instance(instance);
+}
+
+void* lum_get_glfw_ptr(const LumRenderer) { return nullptr; }
+
+bool lum_get_should_close(const LumRenderer instance) {
+    Renderer* r = get(instance);
+    return r ? r->should_close : true;
+}
+
+bool lum_set_should_close(const LumRenderer instance, bool should_close) {
+    Renderer* r = get(instance);
+    if (!r) return should_close;
+    r->should_close = should_close;
+    return r->should_close;
+}
+
+void lum_wait_idle(LumRenderer instance) {
+    Renderer* r = get(instance);
+    if (r && r->ctx) lumx_context_sync(r->ctx);
+}
+
+bool lum_is_profiling_enabled(const LumRenderer instance) { return get(instance) != nullptr; }
+
+int lum_get_current_timestamp_count(const LumRenderer instance) { return get(instance) ? 4 : 0; }
+
+void lum_get_timestamp_name(const LumRenderer instance, int index, char* out_name, size_t name_length) {
+    static const char* names[4] = {"start_frame", "map", "update_radiance", "glossy"};
+    if (!out_name || name_length == 0) return;
+    const char* n = (get(instance) && index >= 0 && index < 4) ? names[index] : "";
+    std::strncpy(out_name, n, name_length - 1);
+    out_name[name_length - 1] = '\0';
+}
+
+double lum_get_timestamp(const LumRenderer instance, int index) {
+    Renderer* r = get(instance);
+    return (r && index >= 0 && index < 4) ? r->ts[index] : 0.0;
+}
+
+double lum_get_timestamp_difference(const LumRenderer instance, int index_start, int index_end) {
+    return lum_get_timestamp(instance, index_end) - lum_get_timestamp(instance, index_start);
+}
+
+void lumx_lum_set_materials(LumRenderer instance, const float* materials) {
+    Renderer* r = get(instance);
+    if (r && materials) std::memcpy(r->mats.data(), materials, sizeof(float) * 256 * 6);
+}
+
+void lumx_lum_request_radiance_update(LumRenderer instance, int x, int y, int z) {
+    Renderer* r = get(instance);
+    if (!r) return;
+    const int* ws = r->settings.world_size;
+    if (x < 0 || y < 0 || z < 0 || x >= ws[0] || y >= ws[1] || z >= ws[2]) return;
+    r->special.insert(r->special.end(), {x, y, z, 0});
+}
+
+void lumx_lum_set_gbuffer(LumRenderer instance, const uint8_t* matnorm_dev, const float* depth_dev, int width,
+                          int height, float* out_dev) {
+    Renderer* r = get(instance);
+    if (!r) return;
+    r->gb_matnorm = matnorm_dev;
+    r->gb_depth = depth_dev;
+    r->gb_out = out_dev;
+    r->gb_w = width;
+    r->gb_h = height;
+}
+
+void* lumx_lum_world(LumRenderer instance) {
+    Renderer* r = get(instance);
+    return r ? (void*)r->world : nullptr;
+}
+
+}  // extern "C"

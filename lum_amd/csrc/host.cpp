@@ -1,0 +1,876 @@
+// host.cpp - host runtime of liblumx: contexts, world/palette/material/model
+// state in HBM, the host logic of the hot passes and the C ABI of lumx.h.
+//
+// Host logic restated from the reference (paths relative to it):
+//   blockify_mesh / start_blockify / end_blockify   src/renderer/src/internal_render.cpp:977-1097
+//   get_shift                                       src/renderer/src/internal_render.cpp:989-1013
+//   map_mesh push constants and dispatch            src/renderer/src/internal_render.cpp:1485-1518
+//   exec_copies                                     src/renderer/src/internal_render.cpp:1375-1476
+//   update_radiance probe selection                 src/renderer/src/internal_render.cpp:1142-1168
+//   Camera::updateCamera + UBO fill                 src/renderer/src/internal_render.cpp:923-934,1580-1592
+//   updateBlockPalette layout transpose             src/renderer/src/load_stuff.cpp:36-86
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lumx.h"
+#include "lumx_internal.hpp"
+
+using namespace lumx;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(LUMX_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+struct lumx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<int, float4*> fib_tables;  // rays -> device table of per-lane directions
+    std::mutex mu;
+};
+
+struct lumx_model {
+    lumx_context* ctx = nullptr;
+    int size[3] = {0, 0, 0};
+    uint8_t* voxels = nullptr;  // device, x-fastest
+};
+
+struct lumx_world {
+    lumx_context* ctx = nullptr;
+    lumx_world_desc desc{};
+    int n_blocks = 0;
+    std::vector<int16_t> origin;   // CPU origin world (Lum's origin_world)
+    std::vector<int16_t> current;  // blockified world of the last map
+    std::vector<int32_t> copy_src; // per palette slot: static source block
+    int counter = 0;               // palette slots in use (paletteCounter)
+    bool block0_empty = true;
+    std::vector<uint8_t> static_palette;  // host copy of the static slots
+    int16_t* blocks = nullptr;     // device blockified world
+    uint8_t* palette = nullptr;    // device palette slots
+    float* mats = nullptr;         // device materials
+    int32_t* copy_src_dev = nullptr;
+    void* field[2] = {nullptr, nullptr};
+    int cur = 0;
+    size_t texel_bytes = 16;
+    hipEvent_t ev[3][2] = {};
+    bool ev_valid[3] = {false, false, false};
+};
+
+static hipStream_t pick_stream(lumx_context* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+// ---------------------------------------------------------------------------
+// host math: quaternion -> matrix, AABB of get_shift, inverse, camera
+// ---------------------------------------------------------------------------
+namespace {
+
+// Rotation matrix of a unit quaternion (w, x, y, z), FP32 as Lum's
+// toMat4(quat); m[c][r] column-major.
+void quat_to_mat(const float q[4], float m[4][4]) {
+    const float w = q[0], x = q[1], y = q[2], z = q[3];
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xz = x * z, xy = x * y, yz = y * z;
+    const float wx = w * x, wy = w * y, wz = w * z;
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) m[c][r] = 0.0f;
+    m[0][0] = 1.0f - 2.0f * (yy + zz);
+    m[0][1] = 2.0f * (xy + wz);
+    m[0][2] = 2.0f * (xz - wy);
+    m[1][0] = 2.0f * (xy - wz);
+    m[1][1] = 1.0f - 2.0f * (xx + zz);
+    m[1][2] = 2.0f * (yz + wx);
+    m[2][0] = 2.0f * (xz + wy);
+    m[2][1] = 2.0f * (yz - wx);
+    m[2][2] = 1.0f - 2.0f * (xx + yy);
+    m[3][3] = 1.0f;
+}
+
+// General 4x4 inverse (Gauss-Jordan, partial pivoting) in double; m[c][r].
+bool invert4(const double a[4][4], double out[4][4]) {
+    // Work row-major: A[r][c] = a[c][r].
+    double A[4][4], inv[4][4];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) A[r][c] = a[c][r];
+    // Gauss-Jordan with partial pivoting.
+    double M[4][8];
+    for (int r = 0; r < 4; ++r) {
+        for (int c = 0; c < 4; ++c) M[r][c] = A[r][c];
+        for (int c = 0; c < 4; ++c) M[r][4 + c] = (r == c) ? 1.0 : 0.0;
+    }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
+        if (M[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int c = 0; c < 8; ++c) std::swap(M[piv][c], M[col][c]);
+        double d = M[col][col];
+        for (int c = 0; c < 8; ++c) M[col][c] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            double f = M[r][col];
+            if (f == 0.0) continue;
+            for (int c = 0; c < 8; ++c) M[r][c] -= f * M[col][c];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) inv[r][c] = M[r][4 + c];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[c][r] = inv[r][c];
+    return true;
+}
+
+// M * (x, y, z, 1) in double, summed (c0*x + c1*y) + (c2*z + c3*1), then
+// rounded to float (get_shift's vec3(trans * vec4(corner, 1.0))).
+void xform_point(const double m[4][4], const float p[3], float out[3]) {
+    for (int r = 0; r < 3; ++r) {
+        double a = m[0][r] * (double)p[0] + m[1][r] * (double)p[1];
+        double b = m[2][r] * (double)p[2] + m[3][r] * 1.0;
+        out[r] = (float)(a + b);
+    }
+}
+
+void dnormalize(double v[3]) {
+    double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] /= l; v[1] /= l; v[2] /= l;
+}
+void fnormalize(float v[3]) {
+    float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] /= l; v[1] /= l; v[2] /= l;
+}
+void dcross(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - b[1] * a[2];
+    o[1] = a[2] * b[0] - b[2] * a[0];
+    o[2] = a[0] * b[1] - b[0] * a[1];
+}
+
+// Per-lane ray directions of sphericalFibonacci(i, n) (radiance.comp:293-306):
+// FP32 operands, cos/sin of the FP32 angle correctly rounded.
+void fib_table(int n, std::vector<float4>& t) {
+    t.resize(n);
+    const float PI = 3.1415926535897932384626433832795f;
+    const float PHI = std::sqrt(5.0f) * 0.5f + 0.5f;
+    for (int i = 0; i < n; ++i) {
+        float fi = (float)i;
+        float a = fi * (PHI - 1.0f);
+        float frac = a - std::floor(a);
+        float phi = 2.0f * PI * frac;
+        float ct = 1.0f - (2.0f * fi + 1.0f) * (1.0f / (float)n);
+        float s2 = 1.0f - ct * ct;
+        s2 = std::min(std::max(s2, 0.0f), 1.0f);
+        float st = std::sqrt(s2);
+        float c = (float)std::cos((double)phi), s = (float)std::sin((double)phi);
+        t[i] = make_float4(c * st, s * st, ct, 0.0f);
+    }
+}
+
+bool valid_rays(int r) {
+    if (r <= 0 || r > 1024) return false;
+    if (r <= 64) return (r & (r - 1)) == 0;
+    return r % 64 == 0;
+}
+
+}  // namespace
+
+static int get_fib(lumx_context* c, int rays, const float4** out) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->fib_tables.find(rays);
+    if (it != c->fib_tables.end()) {
+        *out = it->second;
+        return LUMX_OK;
+    }
+    std::vector<float4> t;
+    fib_table(rays, t);
+    float4* d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(float4) * rays));
+    HIP_TRY(hipMemcpy(d, t.data(), sizeof(float4) * rays, hipMemcpyHostToDevice));
+    c->fib_tables[rays] = d;
+    *out = d;
+    return LUMX_OK;
+}
+
+static WorldView view_of(const lumx_world* w) {
+    WorldView v;
+    v.blocks = w->blocks;
+    v.palette = w->palette;
+    v.mats = w->mats;
+    v.wx = w->desc.size[0];
+    v.wy = w->desc.size[1];
+    v.wz = w->desc.size[2];
+    v.palette_slots = w->desc.palette_slots;
+    v.block0_empty = w->block0_empty ? 1 : 0;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* lumx_last_error(void) { return g_last_error.c_str(); }
+const char* lumx_version(void) { return "lumx 0.1 (gfx950)"; }
+
+int lumx_context_create(int device, lumx_context_t* out) {
+    if (!out) return fail(LUMX_EINVAL, "null output");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LUMX_EINVAL, "device %d out of range (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    lumx_context* c = new lumx_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(LUMX_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return LUMX_OK;
+}
+
+int lumx_context_destroy(lumx_context_t c) {
+    if (!c) return fail(LUMX_EINVAL, "null context");
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->fib_tables) hipFree(kv.second);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return LUMX_OK;
+}
+
+int lumx_context_sync(lumx_context_t c) {
+    if (!c) return fail(LUMX_EINVAL, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return LUMX_OK;
+}
+
+void* lumx_context_stream(lumx_context_t c) { return c ? (void*)c->stream : nullptr; }
+
+// ---- world ----
+static void world_free(lumx_world* w) {
+    hipFree(w->blocks);
+    hipFree(w->palette);
+    hipFree(w->mats);
+    hipFree(w->copy_src_dev);
+    hipFree(w->field[0]);
+    hipFree(w->field[1]);
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < 2; ++k)
+            if (w->ev[p][k]) hipEventDestroy(w->ev[p][k]);
+}
+
+int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* out) {
+    if (!c || !d || !out) return fail(LUMX_EINVAL, "null argument");
+    for (int a = 0; a < 3; ++a)
+        if (d->size[a] <= 0 || d->size[a] > 4096) return fail(LUMX_EINVAL, "world size[%d]=%d invalid", a, d->size[a]);
+    long long nb = (long long)d->size[0] * d->size[1] * d->size[2];
+    if (nb > (1ll << 28)) return fail(LUMX_EINVAL, "world too large (%lld blocks)", nb);
+    if (d->static_blocks < 1 || d->static_blocks > 32767)
+        return fail(LUMX_EINVAL, "static_blocks=%d invalid", d->static_blocks);
+    if (d->radiance_format < 0 || d->radiance_format > 2)
+        return fail(LUMX_EINVAL, "radiance_format=%d invalid", d->radiance_format);
+    HIP_TRY(hipSetDevice(c->device));
+    lumx_world* w = new lumx_world();
+    w->ctx = c;
+    w->desc = *d;
+    w->n_blocks = (int)nb;
+    long long slots = d->palette_slots;
+    if (slots <= 0) slots = std::min<long long>(nb + d->static_blocks, 32768);
+    if (slots < d->static_blocks || slots > 32768) {
+        delete w;
+        return fail(LUMX_EINVAL, "palette_slots=%lld invalid", slots);
+    }
+    w->desc.palette_slots = (int)slots;
+    w->texel_bytes = d->radiance_format == LUMX_RADIANCE_F32 ? 16 : 8;
+    w->origin.assign(nb, 0);
+    w->current.assign(nb, 0);
+    w->copy_src.assign(slots, 0);
+    w->static_palette.assign((size_t)d->static_blocks * 4096, 0);
+    w->counter = d->static_blocks;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    chk(hipMalloc(&w->blocks, nb * sizeof(int16_t)));
+    chk(hipMalloc(&w->palette, (size_t)slots * 4096));
+    chk(hipMalloc(&w->mats, 256 * 6 * sizeof(float)));
+    chk(hipMalloc(&w->copy_src_dev, (size_t)slots * sizeof(int32_t)));
+    chk(hipMalloc(&w->field[0], nb * w->texel_bytes));
+    chk(hipMalloc(&w->field[1], nb * w->texel_bytes));
+    if (e == hipSuccess) {
+        chk(hipMemset(w->blocks, 0, nb * sizeof(int16_t)));
+        chk(hipMemset(w->palette, 0, (size_t)slots * 4096));
+        chk(hipMemset(w->mats, 0, 256 * 6 * sizeof(float)));
+        chk(hipMemset(w->copy_src_dev, 0, (size_t)slots * sizeof(int32_t)));
+        chk(hipMemset(w->field[0], 0, nb * w->texel_bytes));
+        chk(hipMemset(w->field[1], 0, nb * w->texel_bytes));
+        for (int p = 0; p < 3; ++p)
+            for (int k = 0; k < 2; ++k) chk(hipEventCreate(&w->ev[p][k]));
+    }
+    if (e != hipSuccess) {
+        world_free(w);
+        delete w;
+        if (e == hipErrorOutOfMemory) return fail(LUMX_ENOMEM, "device allocation failed");
+        return fail(LUMX_EDEVICE, "world allocation: %s", hipGetErrorString(e));
+    }
+    *out = w;
+    return LUMX_OK;
+}
+
+int lumx_world_destroy(lumx_world_t w) {
+    if (!w) return fail(LUMX_EINVAL, "null world");
+    hipSetDevice(w->ctx->device);
+    hipDeviceSynchronize();
+    world_free(w);
+    delete w;
+    return LUMX_OK;
+}
+
+int lumx_world_info(lumx_world_t w, lumx_world_desc* out) {
+    if (!w || !out) return fail(LUMX_EINVAL, "null argument");
+    *out = w->desc;
+    return LUMX_OK;
+}
+
+static int check_block_ids(const lumx_world* w, const int16_t* b, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (b[i] < 0 || b[i] >= w->desc.palette_slots)
+            return fail(LUMX_EINVAL, "block id %d at %zu outside the palette [0, %d)", (int)b[i], i,
+                        w->desc.palette_slots);
+    return LUMX_OK;
+}
+
+int lumx_world_set_blocks(lumx_world_t w, const int16_t* blocks) {
+    if (!w || !blocks) return fail(LUMX_EINVAL, "null argument");
+    int rc = check_block_ids(w, blocks, w->n_blocks);
+    if (rc) return rc;
+    std::memcpy(w->origin.data(), blocks, (size_t)w->n_blocks * 2);
+    return LUMX_OK;
+}
+
+int lumx_world_get_blocks(lumx_world_t w, int16_t* blocks, int blockified) {
+    if (!w || !blocks) return fail(LUMX_EINVAL, "null argument");
+    std::memcpy(blocks, blockified ? w->current.data() : w->origin.data(), (size_t)w->n_blocks * 2);
+    return LUMX_OK;
+}
+
+static bool in_world(const lumx_world* w, int x, int y, int z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < w->desc.size[0] && y < w->desc.size[1] && z < w->desc.size[2];
+}
+
+int lumx_world_set_block(lumx_world_t w, int x, int y, int z, int16_t b) {
+    if (!w) return fail(LUMX_EINVAL, "null world");
+    if (!in_world(w, x, y, z)) return fail(LUMX_EINVAL, "block (%d,%d,%d) outside the world", x, y, z);
+    if (b < 0 || b >= w->desc.palette_slots) return fail(LUMX_EINVAL, "block id %d invalid", (int)b);
+    w->origin[x + w->desc.size[0] * (y + w->desc.size[1] * z)] = b;
+    return LUMX_OK;
+}
+
+int lumx_world_get_block(lumx_world_t w, int x, int y, int z, int16_t* out) {
+    if (!w || !out) return fail(LUMX_EINVAL, "null argument");
+    if (!in_world(w, x, y, z)) return fail(LUMX_EINVAL, "block (%d,%d,%d) outside the world", x, y, z);
+    *out = w->origin[x + w->desc.size[0] * (y + w->desc.size[1] * z)];
+    return LUMX_OK;
+}
+
+int lumx_world_set_palette(lumx_world_t w, int first, int count, const uint8_t* vox, int layout) {
+    if (!w || (!vox && count > 0)) return fail(LUMX_EINVAL, "null argument");
+    if (first < 0 || count < 0 || first + count > w->desc.static_blocks)
+        return fail(LUMX_EINVAL, "palette range [%d, %d) outside the static palette [0, %d)", first, first + count,
+                    w->desc.static_blocks);
+    if (layout != LUMX_PALETTE_XYZ_XFAST && layout != LUMX_PALETTE_REF_ZFAST)
+        return fail(LUMX_EINVAL, "palette layout %d invalid", layout);
+    std::vector<uint8_t> tmp((size_t)count * 4096);
+    for (int b = 0; b < count; ++b) {
+        const uint8_t* src = vox + (size_t)b * 4096;
+        uint8_t* dst = tmp.data() + (size_t)b * 4096;
+        if (layout == LUMX_PALETTE_XYZ_XFAST) {
+            std::memcpy(dst, src, 4096);
+        } else {
+            // BlockVoxels[x][y][z]: z fastest in memory.
+            for (int x = 0; x < 16; ++x)
+                for (int y = 0; y < 16; ++y)
+                    for (int z = 0; z < 16; ++z) dst[x + 16 * y + 256 * z] = src[(x * 16 + y) * 16 + z];
+        }
+    }
+    std::memcpy(w->static_palette.data() + (size_t)first * 4096, tmp.data(), tmp.size());
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    bool e0 = true;
+    for (int i = 0; i < 4096; ++i)
+        if (w->static_palette[i]) { e0 = false; break; }
+    w->block0_empty = e0;
+    HIP_TRY(hipMemcpy(w->palette + (size_t)first * 4096, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    return LUMX_OK;
+}
+
+int lumx_world_get_palette(lumx_world_t w, int first, int count, uint8_t* out) {
+    if (!w || (!out && count > 0)) return fail(LUMX_EINVAL, "null argument");
+    if (first < 0 || count < 0 || first + count > w->desc.palette_slots)
+        return fail(LUMX_EINVAL, "palette range outside [0, %d)", w->desc.palette_slots);
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    HIP_TRY(hipStreamSynchronize(w->ctx->stream));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, w->palette + (size_t)first * 4096, (size_t)count * 4096, hipMemcpyDeviceToHost));
+    return LUMX_OK;
+}
+
+int lumx_world_set_materials(lumx_world_t w, const float* m) {
+    if (!w || !m) return fail(LUMX_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(w->mats, m, 256 * 6 * sizeof(float), hipMemcpyHostToDevice));
+    return LUMX_OK;
+}
+
+static uint16_t host_f32_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float host_bf16_to_f32(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+int lumx_world_write_radiance(lumx_world_t w, const float* rgba) {
+    if (!w || !rgba) return fail(LUMX_EINVAL, "null argument");
+    size_t n = (size_t)w->n_blocks;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    if (w->desc.radiance_format == LUMX_RADIANCE_F32) {
+        HIP_TRY(hipMemcpy(w->field[w->cur], rgba, n * 16, hipMemcpyHostToDevice));
+        return LUMX_OK;
+    }
+    std::vector<uint16_t> h(n * 4);
+    for (size_t i = 0; i < n * 4; ++i) {
+        if (w->desc.radiance_format == LUMX_RADIANCE_F16) {
+            _Float16 v = (_Float16)rgba[i];
+            std::memcpy(&h[i], &v, 2);
+        } else {
+            h[i] = host_f32_to_bf16(rgba[i]);
+        }
+    }
+    HIP_TRY(hipMemcpy(w->field[w->cur], h.data(), n * 8, hipMemcpyHostToDevice));
+    return LUMX_OK;
+}
+
+int lumx_world_read_radiance(lumx_world_t w, float* rgba) {
+    if (!w || !rgba) return fail(LUMX_EINVAL, "null argument");
+    size_t n = (size_t)w->n_blocks;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    if (w->desc.radiance_format == LUMX_RADIANCE_F32) {
+        HIP_TRY(hipMemcpy(rgba, w->field[w->cur], n * 16, hipMemcpyDeviceToHost));
+        return LUMX_OK;
+    }
+    std::vector<uint16_t> h(n * 4);
+    HIP_TRY(hipMemcpy(h.data(), w->field[w->cur], n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n * 4; ++i) {
+        if (w->desc.radiance_format == LUMX_RADIANCE_F16) {
+            _Float16 v;
+            std::memcpy(&v, &h[i], 2);
+            rgba[i] = (float)v;
+        } else {
+            rgba[i] = host_bf16_to_f32(h[i]);
+        }
+    }
+    return LUMX_OK;
+}
+
+int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes) {
+    if (!w || !ptr || !bytes) return fail(LUMX_EINVAL, "null argument");
+    *ptr = w->field[w->cur];
+    *bytes = (size_t)w->n_blocks * w->texel_bytes;
+    return LUMX_OK;
+}
+
+int lumx_world_upload_blocks(lumx_world_t w, void* stream) {
+    if (!w) return fail(LUMX_EINVAL, "null world");
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    w->current = w->origin;
+    w->counter = w->desc.static_blocks;
+    hipStream_t s = pick_stream(w->ctx, stream);
+    HIP_TRY(hipMemcpyAsync(w->blocks, w->current.data(), (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return LUMX_OK;
+}
+
+int lumx_world_palette_counter(lumx_world_t w, int* out) {
+    if (!w || !out) return fail(LUMX_EINVAL, "null argument");
+    *out = w->counter;
+    return LUMX_OK;
+}
+
+int lumx_world_last_pass_ms(lumx_world_t w, int pass, float* ms) {
+    if (!w || !ms || pass < 0 || pass > 2) return fail(LUMX_EINVAL, "bad argument");
+    if (!w->ev_valid[pass]) return fail(LUMX_EINVAL, "no timed pass of kind %d yet", pass);
+    HIP_TRY(hipEventSynchronize(w->ev[pass][1]));
+    HIP_TRY(hipEventElapsedTime(ms, w->ev[pass][0], w->ev[pass][1]));
+    return LUMX_OK;
+}
+
+// ---- models ----
+int lumx_model_create(lumx_context_t c, const uint8_t* vox, int sx, int sy, int sz, lumx_model_t* out) {
+    if (!c || !vox || !out) return fail(LUMX_EINVAL, "null argument");
+    if (sx <= 0 || sy <= 0 || sz <= 0 || sx > 1024 || sy > 1024 || sz > 1024)
+        return fail(LUMX_EINVAL, "model size %dx%dx%d invalid", sx, sy, sz);
+    HIP_TRY(hipSetDevice(c->device));
+    lumx_model* m = new lumx_model();
+    m->ctx = c;
+    m->size[0] = sx; m->size[1] = sy; m->size[2] = sz;
+    size_t n = (size_t)sx * sy * sz;
+    hipError_t e = hipMalloc(&m->voxels, n);
+    if (e == hipSuccess) e = hipMemcpy(m->voxels, vox, n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(m->voxels);
+        delete m;
+        return fail(e == hipErrorOutOfMemory ? LUMX_ENOMEM : LUMX_EDEVICE, "model upload: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return LUMX_OK;
+}
+
+int lumx_model_destroy(lumx_model_t m) {
+    if (!m) return fail(LUMX_EINVAL, "null model");
+    hipSetDevice(m->ctx->device);
+    hipDeviceSynchronize();
+    hipFree(m->voxels);
+    delete m;
+    return LUMX_OK;
+}
+
+int lumx_model_size(lumx_model_t m, int size[3]) {
+    if (!m || !size) return fail(LUMX_EINVAL, "null argument");
+    size[0] = m->size[0]; size[1] = m->size[1]; size[2] = m->size[2];
+    return LUMX_OK;
+}
+
+int lumx_model_placement(const lumx_transform* t, const int size[3], lumx_placement* out) {
+    if (!t || !size || !out) return fail(LUMX_EINVAL, "null argument");
+    float r[4][4];
+    quat_to_mat(t->rot, r);
+    // translate(I, shift) * rotate == [R | shift] exactly; held in double.
+    double m[4][4];
+    for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < 4; ++k) m[c][k] = (double)r[c][k];
+    m[3][0] = (double)t->shift[0];
+    m[3][1] = (double)t->shift[1];
+    m[3][2] = (double)t->shift[2];
+    m[3][3] = 1.0;
+    // get_shift: AABB of M over the corners of [0, size-1].
+    float box[3] = {(float)(size[0] - 1), (float)(size[1] - 1), (float)(size[2] - 1)};
+    float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float mx[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (int k = 0; k < 8; ++k) {
+        float p[3] = {(k & 1) ? box[0] : 0.0f, (k & 2) ? box[1] : 0.0f, (k & 4) ? box[2] : 0.0f};
+        float q[3];
+        xform_point(m, p, q);
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], q[a]);
+            mx[a] = std::max(mx[a], q[a]);
+        }
+    }
+    double inv[4][4];
+    if (!invert4(m, inv)) return fail(LUMX_EINVAL, "transform is singular");
+    for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < 4; ++k) out->minv[c * 4 + k] = (float)inv[c][k];
+    for (int a = 0; a < 3; ++a) {
+        int lo = (int)std::floor(mn[a]);
+        int hi = (int)std::ceil(mx[a]);
+        out->shift[a] = lo;
+        out->groups[a] = std::max((hi - lo) + 3, 0) / 4;
+        out->aabb_min[a] = mn[a];
+        out->aabb_max[a] = mx[a];
+    }
+    out->shift[3] = 0;
+    return LUMX_OK;
+}
+
+// blockify_mesh for one placed model (internal_render.cpp:1029-1076).
+static int blockify(lumx_world* w, const lumx_placement& p) {
+    int mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = ((int)p.aabb_min[a] - 1) / 16;
+        mx[a] = ((int)p.aabb_max[a] + 1) / 16;
+        mn[a] = std::min(std::max(mn[a], 0), w->desc.size[a] - 1);
+        mx[a] = std::min(std::max(mx[a], 0), w->desc.size[a] - 1);
+    }
+    const int wx = w->desc.size[0], wy = w->desc.size[1];
+    for (int z = mn[2]; z <= mx[2]; ++z)
+        for (int y = mn[1]; y <= mx[1]; ++y)
+            for (int x = mn[0]; x <= mx[0]; ++x) {
+                int16_t& b = w->current[x + wx * (y + wy * z)];
+                if (b < w->desc.static_blocks) {
+                    if (w->counter >= w->desc.palette_slots)
+                        return fail(LUMX_EOVERFLOW, "block palette full (%d slots)", w->desc.palette_slots);
+                    w->copy_src[w->counter] = b;
+                    b = (int16_t)w->counter;
+                    w->counter++;
+                }
+            }
+    return LUMX_OK;
+}
+
+int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                           void* stream) {
+    if (!w || n < 0 || (n > 0 && (!models || !pl))) return fail(LUMX_EINVAL, "bad argument");
+    for (int i = 0; i < n; ++i)
+        if (!models[i] || models[i]->ctx != w->ctx) return fail(LUMX_EINVAL, "model %d invalid", i);
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    // start_blockify: current = origin, counter = static.
+    w->current = w->origin;
+    w->counter = w->desc.static_blocks;
+    for (int i = 0; i < n; ++i) {
+        int rc = blockify(w, pl[i]);
+        if (rc) {
+            w->current = w->origin;
+            w->counter = w->desc.static_blocks;
+            return rc;
+        }
+    }
+    // end_blockify + exec_copies + map.
+    HIP_TRY(hipEventRecord(w->ev[1][0], s));
+    HIP_TRY(hipMemcpyAsync(w->blocks, w->current.data(), (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
+    const int st = w->desc.static_blocks;
+    if (w->counter > st) {
+        HIP_TRY(hipMemcpyAsync(w->copy_src_dev + st, w->copy_src.data() + st, (size_t)(w->counter - st) * 4,
+                               hipMemcpyHostToDevice, s));
+        CopyArgs ca{w->palette, w->copy_src_dev, st, w->counter - st};
+        HIP_TRY(launch_exec_copies(ca, s));
+    }
+    for (int i = 0; i < n; ++i) {
+        const lumx_placement& p = pl[i];
+        MapArgs a;
+        a.blocks = w->blocks;
+        a.palette = w->palette;
+        a.model = models[i]->voxels;
+        a.wx = w->desc.size[0]; a.wy = w->desc.size[1]; a.wz = w->desc.size[2];
+        a.palette_slots = w->desc.palette_slots;
+        a.static_count = st;
+        a.mx = models[i]->size[0]; a.my = models[i]->size[1]; a.mz = models[i]->size[2];
+        a.nx = p.groups[0] * 4; a.ny = p.groups[1] * 4; a.nz = p.groups[2] * 4;
+        a.shift_x = p.shift[0]; a.shift_y = p.shift[1]; a.shift_z = p.shift[2];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) a.m[r * 4 + c] = p.minv[c * 4 + r];
+        HIP_TRY(launch_map(a, s));
+    }
+    HIP_TRY(hipEventRecord(w->ev[1][1], s));
+    w->ev_valid[1] = true;
+    // The host blockified world buffer is re-written by the next map: wait
+    // for the upload to finish before returning.
+    HIP_TRY(hipStreamSynchronize(s));
+    return LUMX_OK;
+}
+
+int lumx_map_models(lumx_world_t w, int n, const lumx_model_t* models, const lumx_transform* t, void* stream) {
+    if (!w || n < 0 || (n > 0 && (!models || !t))) return fail(LUMX_EINVAL, "bad argument");
+    std::vector<lumx_placement> pl(n);
+    for (int i = 0; i < n; ++i) {
+        if (!models[i]) return fail(LUMX_EINVAL, "model %d is null", i);
+        int rc = lumx_model_placement(&t[i], models[i]->size, &pl[i]);
+        if (rc) return rc;
+    }
+    return lumx_map_models_placed(w, n, models, pl.data(), stream);
+}
+
+// ---- radiance ----
+static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
+                         int32_t* keys, int32_t* steps, bool traced, void* stream) {
+    if (!w || !p) return fail(LUMX_EINVAL, "null argument");
+    if (!valid_rays(p->rays))
+        return fail(LUMX_EINVAL, "rays=%d: must be a power of two <= 64 or a multiple of 64 <= 1024", p->rays);
+    if (p->size < 1 || p->shift < 0 || p->shift >= p->size)
+        return fail(LUMX_EINVAL, "size=%d shift=%d invalid", p->size, p->shift);
+    if (!(p->max_dist > 0.0f && p->max_dist <= 65536.0f) || p->max_steps < 0)
+        return fail(LUMX_EINVAL, "max_dist must be in (0, 65536] and max_steps >= 0");
+    if (req && n_req < 0) return fail(LUMX_EINVAL, "n_requests=%d invalid", n_req);
+    int n_total = req ? n_req : w->n_blocks;
+    int n_probes = n_total / p->size;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    const float4* fib = nullptr;
+    int rc = get_fib(w->ctx, p->rays, &fib);
+    if (rc) return rc;
+    RadianceArgs a;
+    a.world = view_of(w);
+    a.field_in = w->field[w->cur];
+    a.field_out = w->field[w->cur ^ 1];
+    a.requests = (const int4*)req;
+    a.fib = fib;
+    a.rays = p->rays;
+    a.n_probes = n_probes;
+    a.time_seed = p->time_seed;
+    a.size = p->size;
+    a.shift = p->shift;
+    a.alpha = p->alpha;
+    a.max_dist = p->max_dist;
+    a.max_steps = p->max_steps;
+    a.ray_keys = keys;
+    a.ray_steps = steps;
+    HIP_TRY(hipEventRecord(w->ev[0][0], s));
+    // Probes outside the request list keep their value in the new field.
+    bool full = (req == nullptr && p->size == 1);
+    if (!full)
+        HIP_TRY(hipMemcpyAsync(a.field_out, a.field_in, (size_t)w->n_blocks * w->texel_bytes,
+                               hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, s));
+    HIP_TRY(hipEventRecord(w->ev[0][1], s));
+    w->ev_valid[0] = true;
+    w->cur ^= 1;
+    return LUMX_OK;
+}
+
+int lumx_radiance_update(lumx_world_t w, const int32_t* req, int n, const lumx_radiance_params* p, void* stream) {
+    return radiance_impl(w, req, n, p, nullptr, nullptr, false, stream);
+}
+
+int lumx_radiance_update_traced(lumx_world_t w, const int32_t* req, int n, const lumx_radiance_params* p,
+                                int32_t* keys, int32_t* steps, void* stream) {
+    return radiance_impl(w, req, n, p, keys, steps, true, stream);
+}
+
+int lumx_select_probes(lumx_world_t w, const int32_t* special, int n_special, int32_t* out, int* n_out) {
+    if (!w || !out || !n_out || n_special < 0 || (n_special > 0 && !special)) return fail(LUMX_EINVAL, "bad argument");
+    const int wx = w->desc.size[0], wy = w->desc.size[1], wz = w->desc.size[2];
+    const std::vector<int16_t>& cw = w->current;
+    std::vector<uint8_t> set((size_t)w->n_blocks, 0);
+    int n = 0;
+    for (int z = 0; z < wz; ++z)
+        for (int y = 0; y < wy; ++y)
+            for (int x = 0; x < wx; ++x) {
+                int sum = 0;
+                for (int dz = -1; dz <= 1; ++dz) {
+                    int tz = std::min(std::max(z + dz, 0), wz - 1);
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        int ty = std::min(std::max(y + dy, 0), wy - 1);
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            int tx = std::min(std::max(x + dx, 0), wx - 1);
+                            sum += cw[tx + wx * (ty + wy * tz)];
+                        }
+                    }
+                }
+                if (sum > 0) {
+                    int32_t* o = out + 4 * n;
+                    o[0] = x; o[1] = y; o[2] = z; o[3] = 0;
+                    set[x + (size_t)wx * (y + (size_t)wy * z)] = 1;
+                    ++n;
+                }
+            }
+    for (int i = 0; i < n_special; ++i) {
+        int x = special[4 * i], y = special[4 * i + 1], z = special[4 * i + 2];
+        if (!in_world(w, x, y, z)) return fail(LUMX_EINVAL, "special request %d outside the world", i);
+        size_t idx = x + (size_t)wx * (y + (size_t)wy * z);
+        if (!set[idx]) {
+            int32_t* o = out + 4 * n;
+            o[0] = x; o[1] = y; o[2] = z; o[3] = special[4 * i + 3];
+            set[idx] = 1;
+            ++n;
+        }
+    }
+    *n_out = n;
+    return LUMX_OK;
+}
+
+// ---- glossy ----
+int lumx_glossy(lumx_world_t w, const uint8_t* matnorm, const float* depth, int width, int height,
+                const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys, void* stream) {
+    if (!w || !matnorm || !depth || !cam || !out) return fail(LUMX_EINVAL, "null argument");
+    if (width <= 0 || height <= 0 || width > 16384 || height > 16384)
+        return fail(LUMX_EINVAL, "frame %dx%d invalid", width, height);
+    if (!(max_dist > 0.0f && max_dist <= 65536.0f) || max_steps < 0)
+        return fail(LUMX_EINVAL, "max_dist must be in (0, 65536] and max_steps >= 0");
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    GlossyArgs a;
+    a.world = view_of(w);
+    a.field = w->field[w->cur];
+    a.matnorm = (const uchar4*)matnorm;
+    a.depth = depth;
+    a.out = (float4*)out;
+    a.keys = keys;
+    a.width = width;
+    a.height = height;
+    for (int k = 0; k < 3; ++k) {
+        a.campos[k] = cam->campos[k];
+        a.camdir[k] = cam->camdir[k];
+        a.horiz[k] = cam->horiz_scaled[k];
+        a.verti[k] = cam->verti_scaled[k];
+        a.light[k] = cam->light_dir[k];
+    }
+    a.frame_w = cam->frame_size[0];
+    a.frame_h = cam->frame_size[1];
+    a.max_dist = max_dist;
+    a.max_steps = max_steps;
+    HIP_TRY(hipEventRecord(w->ev[2][0], s));
+    HIP_TRY(launch_glossy(a, w->desc.radiance_format, keys != nullptr, s));
+    HIP_TRY(hipEventRecord(w->ev[2][1], s));
+    w->ev_valid[2] = true;
+    return LUMX_OK;
+}
+
+int lumx_camera_default(int frame_w, int frame_h, lumx_camera* out) {
+    if (!out || frame_w <= 0 || frame_h <= 0) return fail(LUMX_EINVAL, "bad argument");
+    // Camera defaults (internal_types.hpp:54-66): position (60, 0, 194),
+    // direction normalize(vec3(0.61, 1.0, -0.8)) in FP32, 5 px per voxel over
+    // a 1920x1080 view.
+    float fdir[3] = {0.61f, 1.0f, -0.8f};
+    fnormalize(fdir);
+    double pos[3] = {60.0, 0.0, 194.0};
+    double dir[3] = {fdir[0], fdir[1], fdir[2]};
+    double view[2] = {1920.0 / 5.0, 1080.0 / 5.0};
+    double plane[3] = {dir[0], dir[1], 0.0};
+    dnormalize(plane);
+    double up[3] = {0.0, 0.0, 1.0};
+    double horiz[3], verti[3];
+    dcross(plane, up, horiz);
+    dnormalize(horiz);
+    dcross(dir, horiz, verti);
+    dnormalize(verti);
+    float fl[3] = {0.5f, 0.5f, -0.9f};
+    fnormalize(fl);
+    for (int k = 0; k < 3; ++k) {
+        out->campos[k] = (float)pos[k];
+        out->camdir[k] = (float)dir[k];
+        out->horiz_scaled[k] = (float)(horiz[k] * view[0] / 2.0);
+        out->verti_scaled[k] = (float)(verti[k] * view[1] / 2.0);
+        out->light_dir[k] = fl[k];
+    }
+    out->frame_size[0] = (float)frame_w;
+    out->frame_size[1] = (float)frame_h;
+    return LUMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+// march.hpp - device-side voxel march shared by the radiance and glossy
+// kernels (gfx950).
+//
+// Semantics follow the reference shaders exactly (paths relative to the
+// reference repository):
+//   GetBlock / voxel_in_palette / GetVoxel  shaders/radiance.comp:97-143
+//   GetMat                                  shaders/radiance.comp:145-157
+//   sample_radiance                         shaders/radiance.comp:161-166
+//   CastRay_fast                            shaders/radiance.comp:183-291
+// Arithmetic rules: no FMA contraction (the pragma below and
+// -ffp-contract=off), IEEE division and sqrt, fminf/fmaxf, float->int
+// truncation toward zero, x % 16 == x & 15 (sign of the divisor), block
+// fetches outside the world return 0 and palette slots outside
+// [0, palette_slots) read as voxel 0.
+//
+// HBM layout consumed here:
+//   blocks   int16[wz][wy][wx]                       (2 B per block)
+//   palette  uint8[slot][z][y][x], 4096 B per slot    (slot-contiguous)
+//   mats     float[256][6] = {r, g, b, a, emission, roughness}
+//   field    texel[wz][wy][wx], texel = float4 / half4 / bf16x4
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lumx {
+
+struct f3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 normalize3(f3 v) {
+    float len = sqrtf(dot3(v, v));
+    return {v.x / len, v.y / len, v.z / len};
+}
+
+struct WorldView {
+    const int16_t* blocks;
+    const uint8_t* palette;
+    const float* mats;
+    int wx, wy, wz;
+    int palette_slots;
+    int block0_empty;  // palette slot 0 is all zero: skip its voxel fetch
+};
+
+// Radiance texel storage formats.
+enum { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2 };
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <int STORE> struct Texel;
+template <> struct Texel<STORE_F32> {
+    typedef float4 type;
+    __device__ __forceinline__ static f3 load(const void* base, size_t i) {
+        float4 v = ((const float4*)base)[i];
+        return {v.x, v.y, v.z};
+    }
+    __device__ __forceinline__ static void store(void* base, size_t i, f3 v) {
+        ((float4*)base)[i] = make_float4(v.x, v.y, v.z, 1.0f);
+    }
+};
+template <> struct Texel<STORE_F16> {
+    __device__ __forceinline__ static f3 load(const void* base, size_t i) {
+        uint2 raw = ((const uint2*)base)[i];
+        _Float16 h[4];
+        __builtin_memcpy(h, &raw, 8);
+        return {(float)h[0], (float)h[1], (float)h[2]};
+    }
+    __device__ __forceinline__ static void store(void* base, size_t i, f3 v) {
+        _Float16 h[4] = {(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)1.0f};
+        uint2 raw;
+        __builtin_memcpy(&raw, h, 8);
+        ((uint2*)base)[i] = raw;
+    }
+};
+template <> struct Texel<STORE_BF16> {
+    __device__ __forceinline__ static f3 load(const void* base, size_t i) {
+        uint2 raw = ((const uint2*)base)[i];
+        return {bf16_to_f32((uint16_t)(raw.x & 0xffffu)), bf16_to_f32((uint16_t)(raw.x >> 16)),
+                bf16_to_f32((uint16_t)(raw.y & 0xffffu))};
+    }
+    __device__ __forceinline__ static void store(void* base, size_t i, f3 v) {
+        uint2 raw;
+        raw.x = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
+        raw.y = (uint32_t)f32_to_bf16(v.z) | ((uint32_t)0x3f80u << 16);
+        ((uint2*)base)[i] = raw;
+    }
+};
+
+// GetBlock + palette fetch for an integer voxel position.
+__device__ __forceinline__ int get_voxel(const WorldView& w, int ix, int iy, int iz, int* blk_out) {
+    int bx = ix / 16, by = iy / 16, bz = iz / 16;
+    int b = 0;
+    if ((unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz)
+        b = w.blocks[bx + w.wx * (by + w.wy * bz)];
+    *blk_out = b;
+    if (b == 0 && w.block0_empty) return 0;
+    if ((unsigned)b >= (unsigned)w.palette_slots) return 0;
+    uint32_t off = ((uint32_t)b << 12) | (uint32_t)((ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8));
+    return w.palette[off];
+}
+
+struct MatF {
+    f3 color;
+    float emit, rough;
+};
+__device__ __forceinline__ MatF get_mat(const WorldView& w, int v) {
+    const float* m = w.mats + v * 6;
+    MatF r;
+    r.color = {m[0], m[1], m[2]};
+    r.emit = m[4];
+    r.rough = m[5];
+    return r;
+}
+
+__device__ __forceinline__ int wrap_i(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// sample_radiance: trilinear at texel coordinate (x + n*16)/16, REPEAT wrap.
+template <int STORE>
+__device__ __forceinline__ f3 sample_radiance(const void* field, int wx, int wy, int wz, f3 x, f3 n) {
+    f3 bp = (x + n * 16.0f) / 16.0f;
+    float fx0 = floorf(bp.x), fy0 = floorf(bp.y), fz0 = floorf(bp.z);
+    float tx = bp.x - fx0, ty = bp.y - fy0, tz = bp.z - fz0;
+    int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+    int x0 = wrap_i(ix, wx), x1 = wrap_i(ix + 1, wx);
+    int y0 = wrap_i(iy, wy), y1 = wrap_i(iy + 1, wy);
+    int z0 = wrap_i(iz, wz), z1 = wrap_i(iz + 1, wz);
+    size_t sy = (size_t)wx, sz = (size_t)wx * wy;
+    f3 t000 = Texel<STORE>::load(field, x0 + sy * y0 + sz * z0);
+    f3 t100 = Texel<STORE>::load(field, x1 + sy * y0 + sz * z0);
+    f3 t010 = Texel<STORE>::load(field, x0 + sy * y1 + sz * z0);
+    f3 t110 = Texel<STORE>::load(field, x1 + sy * y1 + sz * z0);
+    f3 t001 = Texel<STORE>::load(field, x0 + sy * y0 + sz * z1);
+    f3 t101 = Texel<STORE>::load(field, x1 + sy * y0 + sz * z1);
+    f3 t011 = Texel<STORE>::load(field, x0 + sy * y1 + sz * z1);
+    f3 t111 = Texel<STORE>::load(field, x1 + sy * y1 + sz * z1);
+    float ux = 1.0f - tx, uy = 1.0f - ty, uz = 1.0f - tz;
+    f3 c00 = t000 * ux + t100 * tx;
+    f3 c10 = t010 * ux + t110 * tx;
+    f3 c01 = t001 * ux + t101 * tx;
+    f3 c11 = t011 * ux + t111 * tx;
+    f3 c0 = c00 * uy + c10 * ty;
+    f3 c1 = c01 * uy + c11 * ty;
+    return c0 * uz + c1 * tz;
+}
+
+struct RayHit {
+    int hit;
+    int voxel;
+    float fraction;
+    f3 normal;
+    int coarse;
+    int refine;
+    int key;
+};
+
+// CastRay_fast: coarse 0.5-voxel march with empty-block skipping, then up to
+// five voxel-DDA refinement steps from origin + dir*(fraction - 1.5).
+__device__ __forceinline__ RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_steps) {
+    RayHit r;
+    r.hit = 0;
+    r.voxel = 0;
+    r.fraction = 0.0f;
+    r.normal = {0.0f, 0.0f, 0.0f};
+    r.coarse = 0;
+    r.refine = 0;
+    r.key = 0;
+    const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    const f3 corner = {d.x > 0.0f ? 16.0f : 0.0f, d.y > 0.0f ? 16.0f : 0.0f, d.z > 0.0f ? 16.0f : 0.0f};
+    const float ext_x = (float)(w.wx * 16), ext_y = (float)(w.wy * 16), ext_z = (float)(w.wz * 16);
+    float fraction = 0.0f;
+    int it = 0;
+    int cur = 0;
+    while (true) {
+        fraction += 0.5f;
+        f3 pos = o + d * fraction;
+        int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
+        int blk;
+        cur = get_voxel(w, ix, iy, iz, &blk);
+        r.coarse++;
+        if (blk == 0) {
+            f3 box = {(float)((ix / 16) * 16) + corner.x, (float)((iy / 16) * 16) + corner.y,
+                      (float)((iz / 16) * 16) + corner.z};
+            f3 tmp = (-pos) * inv;
+            f3 bc = box * inv + tmp;
+            float f = fminf(bc.x, fminf(bc.y, bc.z));
+            f = fmaxf(f, 0.01f);
+            fraction += f;
+        }
+        if (cur != 0) {
+            r.hit = 1;
+            break;
+        }
+        if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || pos.x >= ext_x || pos.y >= ext_y || pos.z >= ext_z)
+            break;
+        if (fraction >= max_dist) break;
+        if (max_steps > 0 && ++it >= max_steps) break;
+    }
+    if (!r.hit) return r;
+
+    f3 bh = o + d * (fraction - 1.5f);
+    int sx = d.x > 0.0f ? 1 : -1, sy = d.y > 0.0f ? 1 : -1, sz = d.z > 0.0f ? 1 : -1;
+    f3 fl = {floorf(bh.x), floorf(bh.y), floorf(bh.z)};
+    f3 c1 = {(fl.x - bh.x) / d.x, (fl.y - bh.y) / d.y, (fl.z - bh.z) / d.z};
+    f3 c2 = {(fl.x - bh.x) / d.x + 1.0f / d.x, (fl.y - bh.y) / d.y + 1.0f / d.y,
+             (fl.z - bh.z) / d.z + 1.0f / d.z};
+    f3 tMax = {fmaxf(c1.x, c2.x), fmaxf(c1.y, c2.y), fmaxf(c1.z, c2.z)};
+    f3 tDelta = {1.0f / fabsf(d.x), 1.0f / fabsf(d.y), 1.0f / fabsf(d.z)};
+    int vx = (int)bh.x, vy = (int)bh.y, vz = (int)bh.z;
+    f3 st = {0.0f, 0.0f, 0.0f};
+    int blk;
+    int cv = get_voxel(w, vx, vy, vz, &blk);
+    r.refine = 1;
+    int iters = 0;
+    while ((iters++ <= 4) && (cv == 0)) {
+        st.x = (tMax.x <= tMax.y && tMax.x <= tMax.z) ? 1.0f : 0.0f;
+        st.y = (tMax.x >= tMax.y && tMax.y <= tMax.z) ? 1.0f : 0.0f;
+        st.z = (tMax.z <= tMax.x && tMax.z <= tMax.y) ? 1.0f : 0.0f;
+        vx += sx * (int)st.x;
+        vy += sy * (int)st.y;
+        vz += sz * (int)st.z;
+        tMax = {tMax.x + tDelta.x * st.x, tMax.y + tDelta.y * st.y, tMax.z + tDelta.z * st.z};
+        cv = get_voxel(w, vx, vy, vz, &blk);
+        r.refine++;
+    }
+    r.normal = -(f3{(float)sx, (float)sy, (float)sz} * st);
+    f3 tf = tMax - tDelta;
+    r.fraction = dot3(tf, st);
+    r.voxel = cv;
+    r.key = 1 + ((vx + 512) & 1023) + (((vy + 512) & 1023) << 10) + (((vz + 512) & 1023) << 20);
+    return r;
+}
+
+// globalLightDir of radiance.comp:42, normalize(vec3(0.5, 0.5, -0.9)) in FP32.
+__device__ __forceinline__ f3 radiance_light_dir() { return normalize3(f3{0.5f, 0.5f, -0.9f}); }
+
+}  // namespace lumx
