@@ -61,7 +61,13 @@ def run_plain(args):
     from lum_amd import synth
     from lum_amd.api import Context, World, radiance_params
 
-    cfg = synth.PLAIN
+    cfg = dict(synth.PLAIN)
+    if args.size:
+        cfg["size"] = tuple(int(v) for v in args.size.split(","))
+    if args.rays:
+        cfg["rays"] = args.rays
+    if args.max_steps is not None:
+        cfg["max_steps"] = args.max_steps
     rng = np.random.default_rng(cfg["seed"])
     size = cfg["size"]
     blocks = synth.make_world(size, rng, cfg["n_palette"])
@@ -83,6 +89,7 @@ def run_plain(args):
         return radiance_params(rays=rays, time_seed=frame, alpha=cfg["alpha"], max_dist=cfg["max_dist"],
                                max_steps=cfg["max_steps"])
 
+    tuning, tune_ms = w.autotune(params(0), reps=5)
     # algorithmic bytes of one launch (traced, untimed)
     keys, steps = w.radiance_update(None, params(0), traced=True)
     torch.cuda.synchronize()
@@ -108,7 +115,7 @@ def run_plain(args):
     value = rays_per_step * args.steps / (total_ms / 1e3)
     achieved = bytes_per_launch / (kern_ms / 1e3) / 1e9
     out = {
-        "metric": "probe-rays/s (plain radiance update, 48x48x16 world, 64 rays/probe)",
+        "metric": "probe-rays/s (plain radiance update, %dx%dx%d world, %d rays/probe)" % (size + (rays,)),
         "value": value,
         "unit": "probe-rays/s",
         "n_gpus": 1,
@@ -120,14 +127,24 @@ def run_plain(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Perlin-terrain world, random palette and materials)",
-        "config": {"workload": "plain radiance update: 48x48x16 blocks, 64-block palette, 36864 probes x 64 "
-                               "rays, max 64 steps, alpha 0.1, FP32 field",
+        "config": {"workload": "plain radiance update: %dx%dx%d blocks, 64-block palette, %d probes x %d rays, "
+                               "max %d steps, alpha 0.1, FP32 field" % (size + (n_probes, rays, cfg["max_steps"])),
                    "probes": n_probes, "rays_per_probe": rays},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "radiance_kernel<0,false>", "kernel_ms": kern_ms,
-                     "bytes_per_launch": bytes_per_launch},
+                     "kernel": "radiance_tiled_kernel" if tuning["variant"] == 1 else "radiance_kernel",
+                     "kernel_ms": kern_ms, "bytes_per_launch": bytes_per_launch},
+        "tuning": tuning,
     }
+    if args.copy_calibration_mb > 0:
+        # A plain device copy of known size, for scaling PMC byte counters.
+        n = int(args.copy_calibration_mb * (1 << 20) // 4)
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = n * 4
     w.close()
     ctx.close()
     if not args.no_cpu_baseline:
@@ -144,6 +161,11 @@ def main():
     ap.add_argument("--workload", default="plain", choices=["plain"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--size", default="", help="override world size, e.g. 256,256,32")
+    ap.add_argument("--rays", type=int, default=0)
+    ap.add_argument("--max-steps", type=int, default=None)
+    ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
+                    help="also run a plain device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the hot path has no multi-GPU form here); got --gpus {args.gpus}",

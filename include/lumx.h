@@ -6,7 +6,8 @@
  * plus the world / block-palette / material / model / radiance-buffer state
  * they run on.  Plain C types only: host pointers are marked _host, device
  * (HBM) pointers _dev, and `stream` is a hipStream_t passed as void* (NULL =
- * the context's own stream).
+ * the HIP null stream, as everywhere in HIP; lumx_context_stream() returns the
+ * context's own non-blocking stream).
  *
  * Every function returns LUMX_OK (0) or a negative LUMX_E* status and never
  * aborts; lumx_last_error() returns the message of the calling thread's last
@@ -107,6 +108,17 @@ typedef struct {
     float frame_size[2];    /* framebuffer size in pixels */
 } lumx_camera;
 
+/* Launch shape of the radiance update.  No reference counterpart: it
+ * replaces the fixed 64-lane workgroup of radiance.comp:37,360. */
+typedef struct {
+    int variant;            /* 0 = one thread per ray, a probe per wave; 1 = tiled persistent */
+    int threads;            /* workgroup size of the tiled kernel (multiple of 64, <= 1024) */
+    int probes_per_batch;   /* probes per batch of the tiled kernel (power of two, 1..64) */
+    int lds_block_bits;     /* stage the empty-block bits in LDS when they fit */
+    int lds_palette_bits;   /* stage the palette occupancy bits in LDS when they fit */
+    int wg_per_cu;          /* persistent workgroups per CU, 0 = occupancy limit */
+} lumx_radiance_tuning;
+
 /* ---- context / errors ---- */
 int lumx_context_create(int device, lumx_context_t* out);
 int lumx_context_destroy(lumx_context_t ctx);
@@ -166,6 +178,15 @@ int lumx_radiance_update(lumx_world_t w, const int32_t* requests_dev, int n_requ
 int lumx_radiance_update_traced(lumx_world_t w, const int32_t* requests_dev, int n_requests,
                                 const lumx_radiance_params* p, int32_t* ray_keys_dev,
                                 int32_t* ray_steps_dev, void* stream);
+/* Launch shape used by this world's radiance updates. */
+int lumx_world_set_tuning(lumx_world_t w, const lumx_radiance_tuning* t);
+int lumx_world_get_tuning(lumx_world_t w, lumx_radiance_tuning* out);
+/* Autotuner: times candidate launch shapes of lumx_radiance_update with
+ * these parameters on this world (its state is left unchanged), keeps the
+ * fastest and returns it (either output may be NULL).  No reference
+ * counterpart. */
+int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int reps,
+                           lumx_radiance_tuning* best, float* best_ms);
 /* Probe selection of update_radiance on the current (blockified) world. */
 int lumx_select_probes(lumx_world_t w, const int32_t* special_host, int n_special,
                        int32_t* requests_host, int* n_out);

@@ -48,6 +48,11 @@ class Placement(C.Structure):
                 ("aabb_min", C.c_float * 3), ("aabb_max", C.c_float * 3)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("variant", C.c_int), ("threads", C.c_int), ("probes_per_batch", C.c_int),
+                ("lds_block_bits", C.c_int), ("lds_palette_bits", C.c_int), ("wg_per_cu", C.c_int)]
+
+
 class Camera(C.Structure):
     _fields_ = [("campos", C.c_float * 3), ("camdir", C.c_float * 3), ("horiz_scaled", C.c_float * 3),
                 ("verti_scaled", C.c_float * 3), ("light_dir", C.c_float * 3), ("frame_size", C.c_float * 2)]
@@ -90,6 +95,9 @@ _SIGS = {
     "lumx_glossy": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.c_float, _I, _P, _P, _P]),
     "lumx_camera_default": (_I, [_I, _I, C.POINTER(Camera)]),
     "lumx_world_last_pass_ms": (_I, [_P, _I, C.POINTER(C.c_float)]),
+    "lumx_world_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "lumx_world_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "lumx_autotune_radiance": (_I, [_P, C.POINTER(RadianceParams), _I, C.POINTER(Tuning), C.POINTER(C.c_float)]),
 }
 
 _lib = None

@@ -9,8 +9,10 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (Camera, Placement, RadianceParams, Transform, WorldDesc, check, load, RADIANCE_F32,
+from ._lib import (Camera, Placement, RadianceParams, Transform, Tuning, WorldDesc, check, load, RADIANCE_F32,
                    PALETTE_XYZ_XFAST)
+
+TUNING_FIELDS = ("variant", "threads", "probes_per_batch", "lds_block_bits", "lds_palette_bits", "wg_per_cu")
 
 
 def _stream():
@@ -149,6 +151,24 @@ class World:
         ms = C.c_float()
         check(self.lib.lumx_world_last_pass_ms(self.handle, kind, C.byref(ms)))
         return ms.value
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        check(self.lib.lumx_world_get_tuning(self.handle, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(self.lib.lumx_world_set_tuning(self.handle, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        check(self.lib.lumx_world_get_tuning(self.handle, C.byref(t)))
+        return {k: getattr(t, k) for k in TUNING_FIELDS}
+
+    def autotune(self, params, reps=5):
+        t = Tuning()
+        ms = C.c_float()
+        check(self.lib.lumx_autotune_radiance(self.handle, C.byref(params), reps, C.byref(t), C.byref(ms)))
+        return {k: getattr(t, k) for k in TUNING_FIELDS}, ms.value
 
     # -- passes --
     def map_models(self, models, rots, shifts):

@@ -54,6 +54,7 @@ static int fail(int code, const char* fmt, ...) {
 // ---------------------------------------------------------------------------
 struct lumx_context {
     int device = 0;
+    int n_cu = 256;
     hipStream_t stream = nullptr;
     std::map<int, float4*> fib_tables;  // rays -> device table of per-lane directions
     std::mutex mu;
@@ -84,9 +85,29 @@ struct lumx_world {
     size_t texel_bytes = 16;
     hipEvent_t ev[3][2] = {};
     bool ev_valid[3] = {false, false, false};
+    int prev_counter = 0;               // palette counter of the previous map
+    uint32_t* empty_bits = nullptr;     // device: per block, 1 = id 0
+    uint32_t* pal_bits = nullptr;       // device: per slot, 128 words of occupancy bits
+    unsigned int* batch_counter = nullptr;  // device: tiled kernel work counter (2 words)
+    lumx_radiance_tuning tune{};
 };
 
-static hipStream_t pick_stream(lumx_context* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+static const size_t kLdsBudget = 64 * 1024;
+
+static lumx_radiance_tuning default_tuning() {
+    lumx_radiance_tuning t{};
+    t.variant = 1;
+    t.threads = 256;
+    t.probes_per_batch = 16;
+    t.lds_block_bits = 1;
+    t.lds_palette_bits = 1;
+    t.wg_per_cu = 0;
+    return t;
+}
+
+// NULL is the HIP null stream (as for every HIP API); the context's own
+// non-blocking stream is available through lumx_context_stream().
+static hipStream_t pick_stream(lumx_context*, void* s) { return (hipStream_t)s; }
 
 // ---------------------------------------------------------------------------
 // host math: quaternion -> matrix, AABB of get_shift, inverse, camera
@@ -247,6 +268,8 @@ int lumx_context_create(int device, lumx_context_t* out) {
     HIP_TRY(hipSetDevice(device));
     lumx_context* c = new lumx_context();
     c->device = device;
+    hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (c->n_cu <= 0) c->n_cu = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -283,6 +306,9 @@ static void world_free(lumx_world* w) {
     hipFree(w->copy_src_dev);
     hipFree(w->field[0]);
     hipFree(w->field[1]);
+    hipFree(w->empty_bits);
+    hipFree(w->pal_bits);
+    hipFree(w->batch_counter);
     for (int p = 0; p < 3; ++p)
         for (int k = 0; k < 2; ++k)
             if (w->ev[p][k]) hipEventDestroy(w->ev[p][k]);
@@ -316,6 +342,8 @@ int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* 
     w->copy_src.assign(slots, 0);
     w->static_palette.assign((size_t)d->static_blocks * 4096, 0);
     w->counter = d->static_blocks;
+    w->prev_counter = d->static_blocks;
+    w->tune = default_tuning();
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     chk(hipMalloc(&w->blocks, nb * sizeof(int16_t)));
@@ -324,7 +352,13 @@ int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* 
     chk(hipMalloc(&w->copy_src_dev, (size_t)slots * sizeof(int32_t)));
     chk(hipMalloc(&w->field[0], nb * w->texel_bytes));
     chk(hipMalloc(&w->field[1], nb * w->texel_bytes));
+    chk(hipMalloc(&w->empty_bits, ((nb + 31) / 32) * sizeof(uint32_t)));
+    chk(hipMalloc(&w->pal_bits, (size_t)slots * 128 * sizeof(uint32_t)));
+    chk(hipMalloc(&w->batch_counter, 2 * sizeof(unsigned int)));
     if (e == hipSuccess) {
+        chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
+        chk(hipMemset(w->pal_bits, 0, (size_t)slots * 128 * sizeof(uint32_t)));
+        chk(hipMemset(w->batch_counter, 0, 2 * sizeof(unsigned int)));
         chk(hipMemset(w->blocks, 0, nb * sizeof(int16_t)));
         chk(hipMemset(w->palette, 0, (size_t)slots * 4096));
         chk(hipMemset(w->mats, 0, 256 * 6 * sizeof(float)));
@@ -428,6 +462,8 @@ int lumx_world_set_palette(lumx_world_t w, int first, int count, const uint8_t* 
         if (w->static_palette[i]) { e0 = false; break; }
     w->block0_empty = e0;
     HIP_TRY(hipMemcpy(w->palette + (size_t)first * 4096, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    HIP_TRY(launch_palette_bits(w->palette, first, count, w->pal_bits, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
     return LUMX_OK;
 }
 
@@ -516,13 +552,45 @@ int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes) {
     return LUMX_OK;
 }
 
+// end_blockify + exec_copies (internal_render.cpp:1092-1097,1375-1476): upload
+// the blockified world, rebuild its empty-block bits, fill every slot
+// allocated by blockify with its static source (or zeros) and zero the slots
+// the previous frame allocated beyond the new counter, so every slot at or
+// above the counter reads as air like the reference's cleared palette.
+static int commit_blocks(lumx_world* w, hipStream_t s) {
+    HIP_TRY(hipMemcpyAsync(w->blocks, w->current.data(), (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_block_bits(w->blocks, w->n_blocks, w->empty_bits, s));
+    const int st = w->desc.static_blocks;
+    const int hi = std::max(w->counter, w->prev_counter);
+    for (int k = w->counter; k < hi; ++k) w->copy_src[k] = 0;
+    if (hi > st) {
+        HIP_TRY(hipMemcpyAsync(w->copy_src_dev + st, w->copy_src.data() + st, (size_t)(hi - st) * 4,
+                               hipMemcpyHostToDevice, s));
+        CopyArgs ca{w->palette, w->copy_src_dev, st, hi - st};
+        HIP_TRY(launch_exec_copies(ca, s));
+    }
+    return LUMX_OK;
+}
+
+// Occupancy bits of the dynamic slots after the palette changed on device.
+static int commit_palette_bits(lumx_world* w, hipStream_t s) {
+    const int st = w->desc.static_blocks;
+    const int hi = std::max(w->counter, w->prev_counter);
+    if (hi > st) HIP_TRY(launch_palette_bits(w->palette, st, hi - st, w->pal_bits, s));
+    w->prev_counter = w->counter;
+    return LUMX_OK;
+}
+
 int lumx_world_upload_blocks(lumx_world_t w, void* stream) {
     if (!w) return fail(LUMX_EINVAL, "null world");
     HIP_TRY(hipSetDevice(w->ctx->device));
     w->current = w->origin;
     w->counter = w->desc.static_blocks;
     hipStream_t s = pick_stream(w->ctx, stream);
-    HIP_TRY(hipMemcpyAsync(w->blocks, w->current.data(), (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
+    int rc = commit_blocks(w, s);
+    if (rc) return rc;
+    rc = commit_palette_bits(w, s);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     return LUMX_OK;
 }
@@ -663,14 +731,9 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, co
     }
     // end_blockify + exec_copies + map.
     HIP_TRY(hipEventRecord(w->ev[1][0], s));
-    HIP_TRY(hipMemcpyAsync(w->blocks, w->current.data(), (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
+    int rc = commit_blocks(w, s);
+    if (rc) return rc;
     const int st = w->desc.static_blocks;
-    if (w->counter > st) {
-        HIP_TRY(hipMemcpyAsync(w->copy_src_dev + st, w->copy_src.data() + st, (size_t)(w->counter - st) * 4,
-                               hipMemcpyHostToDevice, s));
-        CopyArgs ca{w->palette, w->copy_src_dev, st, w->counter - st};
-        HIP_TRY(launch_exec_copies(ca, s));
-    }
     for (int i = 0; i < n; ++i) {
         const lumx_placement& p = pl[i];
         MapArgs a;
@@ -687,6 +750,8 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, co
             for (int c = 0; c < 4; ++c) a.m[r * 4 + c] = p.minv[c * 4 + r];
         HIP_TRY(launch_map(a, s));
     }
+    rc = commit_palette_bits(w, s);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(w->ev[1][1], s));
     w->ev_valid[1] = true;
     // The host blockified world buffer is re-written by the next map: wait
@@ -707,8 +772,7 @@ int lumx_map_models(lumx_world_t w, int n, const lumx_model_t* models, const lum
 }
 
 // ---- radiance ----
-static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
-                         int32_t* keys, int32_t* steps, bool traced, void* stream) {
+static int radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p) {
     if (!w || !p) return fail(LUMX_EINVAL, "null argument");
     if (!valid_rays(p->rays))
         return fail(LUMX_EINVAL, "rays=%d: must be a power of two <= 64 or a multiple of 64 <= 1024", p->rays);
@@ -717,14 +781,32 @@ static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lum
     if (!(p->max_dist > 0.0f && p->max_dist <= 65536.0f) || p->max_steps < 0)
         return fail(LUMX_EINVAL, "max_dist must be in (0, 65536] and max_steps >= 0");
     if (req && n_req < 0) return fail(LUMX_EINVAL, "n_requests=%d invalid", n_req);
+    return LUMX_OK;
+}
+
+static int check_tuning(const lumx_world* w, const lumx_radiance_tuning& t, int rays) {
+    if (t.variant != 0 && t.variant != 1) return fail(LUMX_EINVAL, "tuning variant %d invalid", t.variant);
+    if (t.variant == 0) return LUMX_OK;
+    if (t.threads < 64 || t.threads > 1024 || t.threads % 64)
+        return fail(LUMX_EINVAL, "tuning threads %d invalid", t.threads);
+    int P = t.probes_per_batch;
+    if (P < 1 || P > 64 || (P & (P - 1))) return fail(LUMX_EINVAL, "tuning probes_per_batch %d invalid", P);
+    if (t.wg_per_cu < 0) return fail(LUMX_EINVAL, "tuning wg_per_cu invalid");
+    (void)w;
+    (void)rays;
+    return LUMX_OK;
+}
+
+// Launch one radiance update from field[cur] into field[cur ^ 1] with tuning t.
+static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
+                           int32_t* keys, int32_t* steps, bool traced, hipStream_t s,
+                           const lumx_radiance_tuning& tune) {
     int n_total = req ? n_req : w->n_blocks;
     int n_probes = n_total / p->size;
-    HIP_TRY(hipSetDevice(w->ctx->device));
-    hipStream_t s = pick_stream(w->ctx, stream);
     const float4* fib = nullptr;
     int rc = get_fib(w->ctx, p->rays, &fib);
     if (rc) return rc;
-    RadianceArgs a;
+    RadianceArgs a{};
     a.world = view_of(w);
     a.field_in = w->field[w->cur];
     a.field_out = w->field[w->cur ^ 1];
@@ -740,16 +822,127 @@ static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lum
     a.max_steps = p->max_steps;
     a.ray_keys = keys;
     a.ray_steps = steps;
-    HIP_TRY(hipEventRecord(w->ev[0][0], s));
+    a.empty_bits = w->empty_bits;
+    a.pal_bits = w->pal_bits;
+    a.eb_words = (w->n_blocks + 31) / 32;
+    a.pb_words = w->desc.palette_slots * 128;
+    a.batch_counter = w->batch_counter;
     // Probes outside the request list keep their value in the new field.
     bool full = (req == nullptr && p->size == 1);
     if (!full)
         HIP_TRY(hipMemcpyAsync(a.field_out, a.field_in, (size_t)w->n_blocks * w->texel_bytes,
                                hipMemcpyDeviceToDevice, s));
-    HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, s));
+    if (tune.variant == 0) {
+        HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, s));
+        return LUMX_OK;
+    }
+    RadianceTune t;
+    t.threads = tune.threads;
+    t.probes_per_batch = tune.probes_per_batch;
+    t.wg_per_cu = tune.wg_per_cu;
+    t.eb_lds = 0;
+    t.pb_lds = 0;
+    // Stage what fits the LDS budget: block bits first, then palette bits.
+    size_t base = radiance_tiled_lds_bytes(a, t);
+    if (tune.lds_block_bits && base + 4 * (size_t)a.eb_words <= kLdsBudget) {
+        t.eb_lds = 1;
+        base += 4 * (size_t)a.eb_words;
+    }
+    const int used_slots = std::max(w->counter, w->prev_counter);
+    if (tune.lds_palette_bits && base + 4 * (size_t)used_slots * 128 <= kLdsBudget) {
+        t.pb_lds = 1;
+        a.pb_words = used_slots * 128;
+    }
+    if (base > kLdsBudget) return fail(LUMX_EINVAL, "tiled radiance batch does not fit LDS (%zu B)", base);
+    HIP_TRY(launch_radiance_tiled(a, w->desc.radiance_format, traced, t, w->ctx->n_cu, s));
+    return LUMX_OK;
+}
+
+static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
+                         int32_t* keys, int32_t* steps, bool traced, void* stream) {
+    int rc = radiance_check(w, req, n_req, p);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    HIP_TRY(hipEventRecord(w->ev[0][0], s));
+    rc = radiance_launch(w, req, n_req, p, keys, steps, traced, s, w->tune);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(w->ev[0][1], s));
     w->ev_valid[0] = true;
     w->cur ^= 1;
+    return LUMX_OK;
+}
+
+int lumx_world_set_tuning(lumx_world_t w, const lumx_radiance_tuning* t) {
+    if (!w || !t) return fail(LUMX_EINVAL, "null argument");
+    int rc = check_tuning(w, *t, 64);
+    if (rc) return rc;
+    w->tune = *t;
+    return LUMX_OK;
+}
+
+int lumx_world_get_tuning(lumx_world_t w, lumx_radiance_tuning* out) {
+    if (!w || !out) return fail(LUMX_EINVAL, "null argument");
+    *out = w->tune;
+    return LUMX_OK;
+}
+
+int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int reps, lumx_radiance_tuning* best,
+                           float* best_ms) {
+    int rc = radiance_check(w, nullptr, 0, p);
+    if (rc) return rc;
+    if (reps < 1) reps = 1;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = w->ctx->stream;
+    std::vector<lumx_radiance_tuning> cands;
+    lumx_radiance_tuning c0{};
+    c0.variant = 0;
+    cands.push_back(c0);
+    const int threads[] = {256, 512, 1024};
+    const int probes[] = {4, 8, 16, 32};
+    for (int th : threads)
+        for (int P : probes)
+            for (int pb = 0; pb <= 1; ++pb) {
+                if ((long long)P * p->rays > 4096) continue;
+                lumx_radiance_tuning c{};
+                c.variant = 1;
+                c.threads = th;
+                c.probes_per_batch = P;
+                c.lds_block_bits = 1;
+                c.lds_palette_bits = pb;
+                cands.push_back(c);
+            }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best_t = 1e30f;
+    lumx_radiance_tuning best_c = w->tune;
+    for (const lumx_radiance_tuning& c : cands) {
+        // Every trial writes field[cur ^ 1] only; the current field is unchanged.
+        rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c);
+        if (rc) continue;  // shape does not fit this world
+        HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) {
+            rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c);
+            if (rc) break;
+        }
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        if (rc) continue;
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        if (ms < best_t) {
+            best_t = ms;
+            best_c = c;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    w->tune = best_c;
+    if (best) *best = best_c;
+    if (best_ms) *best_ms = best_t;
+    g_last_error.clear();
     return LUMX_OK;
 }
 

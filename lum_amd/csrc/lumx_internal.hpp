@@ -25,6 +25,23 @@ struct RadianceArgs {
     int max_steps;
     int32_t* ray_keys;      // traced launches only
     int32_t* ray_steps;
+    // tiled kernel only
+    const uint32_t* empty_bits;  // per block: 1 = block id 0
+    const uint32_t* pal_bits;    // per slot: 128 words of voxel-occupancy bits
+    int eb_words;
+    int pb_words;
+    int probes_per_batch;        // P: probes per batch (power of two)
+    int n_batches;
+    unsigned int* batch_counter; // 2 words: next batch, finished workgroups (self-resetting)
+};
+
+// Launch shape of the tiled radiance kernel (chosen by the autotuner).
+struct RadianceTune {
+    int threads;          // workgroup size (multiple of 64, <= 1024)
+    int probes_per_batch; // P
+    int eb_lds;           // stage empty-block bits in LDS
+    int pb_lds;           // stage palette occupancy bits in LDS
+    int wg_per_cu;        // persistent workgroups per CU (0 = occupancy limit)
 };
 
 struct MapArgs {
@@ -62,6 +79,12 @@ struct GlossyArgs {
 };
 
 hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, hipStream_t stream);
+hipError_t launch_radiance_tiled(RadianceArgs a, int store, bool traced, const RadianceTune& t, int n_cu,
+                                 hipStream_t stream);
+size_t radiance_tiled_lds_bytes(const RadianceArgs& a, const RadianceTune& t);
+hipError_t launch_block_bits(const int16_t* blocks, int n_blocks, uint32_t* bits, hipStream_t stream);
+hipError_t launch_palette_bits(const uint8_t* palette, int first_slot, int n_slots, uint32_t* bits,
+                               hipStream_t stream);
 hipError_t launch_map(const MapArgs& a, hipStream_t stream);
 hipError_t launch_exec_copies(const CopyArgs& a, hipStream_t stream);
 hipError_t launch_glossy(const GlossyArgs& a, int store, bool traced, hipStream_t stream);

@@ -253,3 +253,117 @@ __device__ __forceinline__ RayHit cast_ray(const WorldView& w, f3 o, f3 d, float
 __device__ __forceinline__ f3 radiance_light_dir() { return normalize3(f3{0.5f, 0.5f, -0.9f}); }
 
 }  // namespace lumx
+
+namespace lumx {
+
+// Occupancy view used by the tiled kernels: per-block "id == 0" bits and
+// per-slot voxel-occupancy bits (1 = voxel != 0), either of which may live
+// in LDS.  A sample needs the block id (global int16) only for blocks whose
+// id is non-zero, and the voxel value only at the refined hit voxel.
+//   empty_bits: bit (x + wx*(y + wy*z)) of word >> 5
+//   pal_bits:   slot * 128 words; voxel index i = x + 16y + 256z -> bit i
+struct OccView {
+    const int16_t* ids;
+    const uint8_t* palette;
+    const uint32_t* eb;
+    const uint32_t* pb;
+    int wx, wy, wz;
+    int palette_slots;
+    int block0_empty;
+
+    __device__ __forceinline__ int block_at(int ix, int iy, int iz) const {
+        int bx = ix / 16, by = iy / 16, bz = iz / 16;
+        if ((unsigned)bx >= (unsigned)wx || (unsigned)by >= (unsigned)wy || (unsigned)bz >= (unsigned)wz) return 0;
+        int lin = bx + wx * (by + wy * bz);
+        if ((eb[lin >> 5] >> (lin & 31)) & 1u) return 0;
+        return ids[lin];
+    }
+    __device__ __forceinline__ bool occupied(int b, int ix, int iy, int iz) const {
+        if (b == 0 && block0_empty) return false;
+        if ((unsigned)b >= (unsigned)palette_slots) return false;
+        int idx = (ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8);
+        return (pb[((uint32_t)b << 7) | (uint32_t)(idx >> 5)] >> (idx & 31)) & 1u;
+    }
+    __device__ __forceinline__ int voxel_value(int b, int ix, int iy, int iz) const {
+        int idx = (ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8);
+        return palette[((uint32_t)b << 12) | (uint32_t)idx];
+    }
+};
+
+// CastRay_fast over an OccView; same arithmetic and results as cast_ray().
+__device__ __forceinline__ RayHit cast_ray_occ(const OccView& w, f3 o, f3 d, float max_dist, int max_steps) {
+    RayHit r;
+    r.hit = 0;
+    r.voxel = 0;
+    r.fraction = 0.0f;
+    r.normal = {0.0f, 0.0f, 0.0f};
+    r.coarse = 0;
+    r.refine = 0;
+    r.key = 0;
+    const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    const f3 corner = {d.x > 0.0f ? 16.0f : 0.0f, d.y > 0.0f ? 16.0f : 0.0f, d.z > 0.0f ? 16.0f : 0.0f};
+    const float ext_x = (float)(w.wx * 16), ext_y = (float)(w.wy * 16), ext_z = (float)(w.wz * 16);
+    float fraction = 0.0f;
+    int it = 0;
+    while (true) {
+        fraction += 0.5f;
+        f3 pos = o + d * fraction;
+        int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
+        int blk = w.block_at(ix, iy, iz);
+        bool occ = w.occupied(blk, ix, iy, iz);
+        r.coarse++;
+        if (blk == 0) {
+            f3 box = {(float)((ix / 16) * 16) + corner.x, (float)((iy / 16) * 16) + corner.y,
+                      (float)((iz / 16) * 16) + corner.z};
+            f3 tmp = (-pos) * inv;
+            f3 bc = box * inv + tmp;
+            float f = fminf(bc.x, fminf(bc.y, bc.z));
+            f = fmaxf(f, 0.01f);
+            fraction += f;
+        }
+        if (occ) {
+            r.hit = 1;
+            break;
+        }
+        if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || pos.x >= ext_x || pos.y >= ext_y || pos.z >= ext_z)
+            break;
+        if (fraction >= max_dist) break;
+        if (max_steps > 0 && ++it >= max_steps) break;
+    }
+    if (!r.hit) return r;
+
+    f3 bh = o + d * (fraction - 1.5f);
+    int sx = d.x > 0.0f ? 1 : -1, sy = d.y > 0.0f ? 1 : -1, sz = d.z > 0.0f ? 1 : -1;
+    f3 fl = {floorf(bh.x), floorf(bh.y), floorf(bh.z)};
+    f3 c1 = {(fl.x - bh.x) / d.x, (fl.y - bh.y) / d.y, (fl.z - bh.z) / d.z};
+    f3 c2 = {(fl.x - bh.x) / d.x + 1.0f / d.x, (fl.y - bh.y) / d.y + 1.0f / d.y,
+             (fl.z - bh.z) / d.z + 1.0f / d.z};
+    f3 tMax = {fmaxf(c1.x, c2.x), fmaxf(c1.y, c2.y), fmaxf(c1.z, c2.z)};
+    f3 tDelta = {1.0f / fabsf(d.x), 1.0f / fabsf(d.y), 1.0f / fabsf(d.z)};
+    int vx = (int)bh.x, vy = (int)bh.y, vz = (int)bh.z;
+    f3 st = {0.0f, 0.0f, 0.0f};
+    int vb = w.block_at(vx, vy, vz);
+    bool cv = w.occupied(vb, vx, vy, vz);
+    r.refine = 1;
+    int iters = 0;
+    while ((iters++ <= 4) && !cv) {
+        st.x = (tMax.x <= tMax.y && tMax.x <= tMax.z) ? 1.0f : 0.0f;
+        st.y = (tMax.x >= tMax.y && tMax.y <= tMax.z) ? 1.0f : 0.0f;
+        st.z = (tMax.z <= tMax.x && tMax.z <= tMax.y) ? 1.0f : 0.0f;
+        vx += sx * (int)st.x;
+        vy += sy * (int)st.y;
+        vz += sz * (int)st.z;
+        tMax = {tMax.x + tDelta.x * st.x, tMax.y + tDelta.y * st.y, tMax.z + tDelta.z * st.z};
+        vb = w.block_at(vx, vy, vz);
+        cv = w.occupied(vb, vx, vy, vz);
+        r.refine++;
+    }
+    r.normal = -(f3{(float)sx, (float)sy, (float)sz} * st);
+    f3 tf = tMax - tDelta;
+    r.fraction = dot3(tf, st);
+    r.voxel = cv ? w.voxel_value(vb, vx, vy, vz) : 0;
+    r.key = 1 + ((vx + 512) & 1023) + (((vy + 512) & 1023) << 10) + (((vz + 512) & 1023) << 20);
+    return r;
+}
+
+}  // namespace lumx

@@ -153,3 +153,281 @@ hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, hipStr
 }
 
 }  // namespace lumx
+
+// ---------------------------------------------------------------------------
+// Tiled persistent radiance kernel.
+//
+// A batch is P consecutive probes x R rays.  Ray q of a batch is direction
+// l = q / P of probe p = q % P, so the 64 lanes of a wave trace 64/P
+// directions from P neighbouring probes: similar lengths and neighbouring
+// addresses.  Per-ray lights go to LDS and each probe's mean is reduced there
+// in the fixed order (butterfly-equivalent tree over min(R,64) rays, chunk
+// sums in chunk order, one division by R).  Workgroups are persistent and
+// take batches from a global counter that the last workgroup resets to 0.
+// LDS holds the material table and, when they fit, the empty-block bits and
+// the palette occupancy bits.
+// ---------------------------------------------------------------------------
+namespace lumx {
+
+size_t radiance_tiled_lds_bytes(const RadianceArgs& a, const RadianceTune& t) {
+    size_t b = 1536 * sizeof(float);                                   // materials
+    b += 3 * sizeof(float) * (size_t)t.probes_per_batch * a.rays;      // per-ray light
+    if (t.eb_lds) b += sizeof(uint32_t) * (size_t)a.eb_words;
+    if (t.pb_lds) b += sizeof(uint32_t) * (size_t)a.pb_words;
+    return b + 16;
+}
+
+template <int STORE, bool TRACED, bool EB_LDS, bool PB_LDS>
+__global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
+    extern __shared__ float smem[];
+    __shared__ int s_batch;
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int R = a.rays, P = a.probes_per_batch, NR = P * R;
+    float* s_mats = smem;
+    float* s_lx = s_mats + 1536;
+    float* s_ly = s_lx + NR;
+    float* s_lz = s_ly + NR;
+    uint32_t* s_eb = (uint32_t*)(s_lz + NR);
+    uint32_t* s_pb = s_eb + (EB_LDS ? a.eb_words : 0);
+
+    for (int i = tid; i < 1536; i += T) s_mats[i] = a.world.mats[i];
+    if (EB_LDS)
+        for (int i = tid; i < a.eb_words; i += T) s_eb[i] = a.empty_bits[i];
+    if (PB_LDS)
+        for (int i = tid; i < a.pb_words; i += T) s_pb[i] = a.pal_bits[i];
+
+    const WorldView& wv = a.world;
+    OccView occ;
+    occ.ids = wv.blocks;
+    occ.palette = wv.palette;
+    occ.eb = EB_LDS ? s_eb : a.empty_bits;
+    occ.pb = PB_LDS ? s_pb : a.pal_bits;
+    occ.wx = wv.wx;
+    occ.wy = wv.wy;
+    occ.wz = wv.wz;
+    // Staged palette bits cover the slots in use; every slot above them is air.
+    occ.palette_slots = PB_LDS ? a.pb_words / 128 : wv.palette_slots;
+    occ.block0_empty = wv.block0_empty;
+    const int C = R < 64 ? R : 64;
+    const uint32_t t = (uint32_t)a.time_seed;
+
+    while (true) {
+        __syncthreads();  // staging done / previous batch fully consumed
+        if (tid == 0) s_batch = (int)atomicAdd(&a.batch_counter[0], 1u);
+        __syncthreads();
+        const int batch = s_batch;
+        if (batch >= a.n_batches) break;
+
+        for (int q = tid; q < NR; q += T) {
+            const int l = q / P, p = q - l * P;
+            const int wg = batch * P + p;
+            f3 light = {0.0f, 0.0f, 0.0f};
+            int px = 0, py = 0, pz = 0;
+            bool valid = false;
+            if (wg < a.n_probes) {
+                if (a.requests) {
+                    int4 rq = a.requests[wg * a.size + a.shift];
+                    px = rq.x;
+                    py = rq.y;
+                    pz = rq.z;
+                } else {
+                    px = wg % wv.wx;
+                    py = (wg / wv.wx) % wv.wy;
+                    pz = wg / (wv.wx * wv.wy);
+                }
+                valid = (unsigned)px < (unsigned)wv.wx && (unsigned)py < (unsigned)wv.wy &&
+                        (unsigned)pz < (unsigned)wv.wz;
+            }
+            if (valid) {
+                const uint32_t gidx = (uint32_t)wg * (uint32_t)R + (uint32_t)l;
+                f3 r0, r1;
+                hash33(gidx + t, t, t, &r0);
+                hash33(gidx + t + 1u, t, t, &r1);
+                const float4 fb = a.fib[l];
+                f3 corner = f3{(float)px, (float)py, (float)pz} * 16.0f;
+                f3 center = corner + f3{8.0f, 8.0f, 8.0f};
+                const f3 half = {0.5f, 0.5f, 0.5f};
+                f3 dir = normalize3(f3{fb.x, fb.y, fb.z} + 0.1f * ((r0 - half) * 2.0f));
+                f3 pos = center + dir * 6.0f;
+                pos = pos + (r1 - half) * 2.0f;
+                if (l & 1) dir = -dir;
+                RayHit h = cast_ray_occ(occ, pos, dir, a.max_dist, a.max_steps);
+                if (h.hit) {
+                    const float* m = s_mats + h.voxel * 6;
+                    const f3 color = {m[0], m[1], m[2]};
+                    const float emit = m[4];
+                    f3 prop = sample_radiance<STORE>(a.field_in, wv.wx, wv.wy, wv.wz, pos + h.fraction * dir,
+                                                     h.normal);
+                    light = light + (prop * color + color * emit);
+                } else {
+                    float gl = -dot3(dir, radiance_light_dir());
+                    if (gl > 0.95f) {
+                        f3 sun = f3{0.9f, 0.9f, 0.6f} * 3.0f;
+                        light = light + gl * sun;
+                    }
+                    f3 sky = f3{0.53f, 0.81f, 0.92f} * 0.1f;
+                    light = light + sky;
+                }
+                if (TRACED) {
+                    size_t ri = (size_t)wg * R + l;
+                    if (a.ray_keys) a.ray_keys[ri] = h.hit ? h.key : 0;
+                    if (a.ray_steps) {
+                        a.ray_steps[2 * ri + 0] = h.coarse;
+                        a.ray_steps[2 * ri + 1] = h.refine;
+                    }
+                }
+            }
+            s_lx[q] = light.x;
+            s_ly[q] = light.y;
+            s_lz[q] = light.z;
+        }
+        __syncthreads();
+        // Tree over rays within chunks of C: light[l] += light[l + s], s = C/2 .. 1.
+        for (int s = C >> 1; s >= 1; s >>= 1) {
+            const int n_pairs = (R / C) * s * P;
+            for (int i = tid; i < n_pairs; i += T) {
+                const int p = i % P;
+                const int rest = i / P;
+                const int l = rest % s;
+                const int c = rest / s;
+                const int q0 = (c * C + l) * P + p, q1 = q0 + s * P;
+                s_lx[q0] = s_lx[q0] + s_lx[q1];
+                s_ly[q0] = s_ly[q0] + s_ly[q1];
+                s_lz[q0] = s_lz[q0] + s_lz[q1];
+            }
+            __syncthreads();
+        }
+        for (int p = tid; p < P; p += T) {
+            const int wg = batch * P + p;
+            if (wg >= a.n_probes) continue;
+            int px, py, pz;
+            if (a.requests) {
+                int4 rq = a.requests[wg * a.size + a.shift];
+                px = rq.x;
+                py = rq.y;
+                pz = rq.z;
+            } else {
+                px = wg % wv.wx;
+                py = (wg / wv.wx) % wv.wy;
+                pz = wg / (wv.wx * wv.wy);
+            }
+            if ((unsigned)px >= (unsigned)wv.wx || (unsigned)py >= (unsigned)wv.wy || (unsigned)pz >= (unsigned)wv.wz)
+                continue;
+            f3 total = {s_lx[p], s_ly[p], s_lz[p]};
+            for (int c = 1; c < R / C; ++c) {
+                const int q = c * C * P + p;
+                total = total + f3{s_lx[q], s_ly[q], s_lz[q]};
+            }
+            f3 mean = total / (float)R;
+            size_t ti = (size_t)px + (size_t)wv.wx * ((size_t)py + (size_t)wv.wy * pz);
+            f3 old = Texel<STORE>::load(a.field_in, ti);
+            old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
+            const float al = a.alpha, bl = 1.0f - a.alpha;
+            f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
+            Texel<STORE>::store(a.field_out, ti, nv);
+        }
+    }
+    // The last workgroup to finish resets the batch counter for the next launch.
+    if (tid == 0) {
+        __threadfence();
+        unsigned int done = atomicAdd(&a.batch_counter[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&a.batch_counter[0], 0u);
+            atomicExch(&a.batch_counter[1], 0u);
+        }
+    }
+}
+
+template <int STORE, bool TRACED, bool EB, bool PB>
+static hipError_t launch_tiled_inst(const RadianceArgs& a, const RadianceTune& t, int n_cu, hipStream_t stream) {
+    const size_t lds = radiance_tiled_lds_bytes(a, t);
+    auto kern = radiance_tiled_kernel<STORE, TRACED, EB, PB>;
+    // Occupancy per (threads, LDS size) for this instantiation, cached.
+    static thread_local int cached_threads = -1;
+    static thread_local size_t cached_lds = 0;
+    static thread_local int cached_per_cu = 0;
+    int per_cu = 0;
+    if (cached_threads == t.threads && cached_lds == lds) {
+        per_cu = cached_per_cu;
+    } else {
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, t.threads, lds);
+        if (e != hipSuccess) return e;
+        cached_threads = t.threads;
+        cached_lds = lds;
+        cached_per_cu = per_cu;
+    }
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    if (t.wg_per_cu > 0 && t.wg_per_cu < per_cu) per_cu = t.wg_per_cu;
+    long long grid = (long long)per_cu * n_cu;
+    if (grid > a.n_batches) grid = a.n_batches;
+    if (grid < 1) return hipSuccess;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(t.threads), lds, stream, a);
+    return hipGetLastError();
+}
+
+template <int STORE, bool TRACED>
+static hipError_t launch_tiled_store(const RadianceArgs& a, const RadianceTune& t, int n_cu, hipStream_t s) {
+    if (t.eb_lds && t.pb_lds) return launch_tiled_inst<STORE, TRACED, true, true>(a, t, n_cu, s);
+    if (t.eb_lds) return launch_tiled_inst<STORE, TRACED, true, false>(a, t, n_cu, s);
+    if (t.pb_lds) return launch_tiled_inst<STORE, TRACED, false, true>(a, t, n_cu, s);
+    return launch_tiled_inst<STORE, TRACED, false, false>(a, t, n_cu, s);
+}
+
+hipError_t launch_radiance_tiled(RadianceArgs a, int store, bool traced, const RadianceTune& t, int n_cu,
+                                 hipStream_t stream) {
+    if (a.n_probes <= 0) return hipSuccess;
+    a.probes_per_batch = t.probes_per_batch;
+    a.n_batches = (a.n_probes + t.probes_per_batch - 1) / t.probes_per_batch;
+    if (traced) {
+        switch (store) {
+            case STORE_F16: return launch_tiled_store<STORE_F16, true>(a, t, n_cu, stream);
+            case STORE_BF16: return launch_tiled_store<STORE_BF16, true>(a, t, n_cu, stream);
+            default: return launch_tiled_store<STORE_F32, true>(a, t, n_cu, stream);
+        }
+    }
+    switch (store) {
+        case STORE_F16: return launch_tiled_store<STORE_F16, false>(a, t, n_cu, stream);
+        case STORE_BF16: return launch_tiled_store<STORE_BF16, false>(a, t, n_cu, stream);
+        default: return launch_tiled_store<STORE_F32, false>(a, t, n_cu, stream);
+    }
+}
+
+// Per-block "id == 0" bits, one thread per 32 blocks.
+__global__ void __launch_bounds__(256) block_bits_kernel(const int16_t* blocks, int n, uint32_t* bits) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int base = w * 32;
+    if (base >= n) return;
+    uint32_t v = 0;
+    for (int k = 0; k < 32 && base + k < n; ++k) v |= (blocks[base + k] == 0 ? 1u : 0u) << k;
+    bits[w] = v;
+}
+
+// Per-slot voxel-occupancy bits: one workgroup of 128 threads per slot, one
+// output word (32 voxels) per thread.
+__global__ void __launch_bounds__(128) palette_bits_kernel(const uint8_t* palette, int first, uint32_t* bits) {
+    const int slot = first + blockIdx.x;
+    const uint8_t* src = palette + ((size_t)slot << 12) + threadIdx.x * 32;
+    const uint4 a0 = ((const uint4*)src)[0];
+    const uint4 a1 = ((const uint4*)src)[1];
+    const uint32_t w[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v |= (((w[k >> 2] >> ((k & 3) * 8)) & 0xffu) != 0 ? 1u : 0u) << k;
+    bits[((size_t)slot << 7) + threadIdx.x] = v;
+}
+
+hipError_t launch_block_bits(const int16_t* blocks, int n_blocks, uint32_t* bits, hipStream_t stream) {
+    int words = (n_blocks + 31) / 32;
+    if (words == 0) return hipSuccess;
+    hipLaunchKernelGGL(block_bits_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, blocks, n_blocks, bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_palette_bits(const uint8_t* palette, int first_slot, int n_slots, uint32_t* bits,
+                               hipStream_t stream) {
+    if (n_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(palette_bits_kernel, dim3(n_slots), dim3(128), 0, stream, palette, first_slot, bits);
+    return hipGetLastError();
+}
+
+}  // namespace lumx

@@ -49,16 +49,26 @@ def full_palette(world, pal):
     return p
 
 
+TUNINGS = {
+    "lane": dict(variant=0),
+    "tiled": dict(variant=1, threads=256, probes_per_batch=16, lds_block_bits=1, lds_palette_bits=1),
+    "tiled_global_bits": dict(variant=1, threads=1024, probes_per_batch=4, lds_block_bits=0, lds_palette_bits=0),
+    "tiled_p32": dict(variant=1, threads=512, probes_per_batch=32, lds_block_bits=1, lds_palette_bits=0),
+}
+
+
+@pytest.mark.parametrize("tuning", sorted(TUNINGS))
 @pytest.mark.parametrize("size,rays,seed,max_steps", [
     ((16, 16, 8), 64, 1, 0),
     ((48, 48, 16), 64, 1234, 64),
     ((24, 20, 12), 32, 7, 0),
     ((20, 24, 10), 128, 8, 96),
 ])
-def test_radiance_all_probes(gpu_ctx, size, rays, seed, max_steps):
+def test_radiance_all_probes(gpu_ctx, size, rays, seed, max_steps, tuning):
     import torch
     blocks, pal, mats, field = make_case(size, seed)
     w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    w.set_tuning(**TUNINGS[tuning])
     try:
         keys, steps = w.radiance_update(None, radiance_params(rays=rays, time_seed=3, alpha=0.1,
                                                               max_steps=max_steps), traced=True)
@@ -78,7 +88,8 @@ def test_radiance_all_probes(gpu_ctx, size, rays, seed, max_steps):
         w.close()
 
 
-def test_radiance_request_list_jacobi(gpu_ctx):
+@pytest.mark.parametrize("tuning", ["lane", "tiled"])
+def test_radiance_request_list_jacobi(gpu_ctx, tuning):
     """Subset of probes with size/shift: unrequested probes keep their value."""
     import torch
     size = (20, 18, 10)
@@ -89,6 +100,7 @@ def test_radiance_request_list_jacobi(gpu_ctx):
     flat = rng.choice(wx * wy * wz, size=n, replace=False)
     req = np.stack([flat % wx, (flat // wx) % wy, flat // (wx * wy), np.zeros(n, int)], 1).astype(np.int32)
     w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    w.set_tuning(**TUNINGS[tuning])
     try:
         req_d = torch.from_numpy(req).cuda()
         w.radiance_update(req_d, radiance_params(rays=64, time_seed=11, size=3, shift=2, alpha=0.25))
@@ -227,4 +239,56 @@ def test_select_probes_matches_oracle(gpu_ctx):
         ref = orc.select_probes(blocks, special[:2])
         assert np.array_equal(got, ref)
     finally:
+        w.close()
+
+
+def test_autotune_keeps_state_and_parity(gpu_ctx):
+    """The autotuner leaves the field unchanged and its pick stays in parity."""
+    import torch
+    size = (24, 24, 12)
+    blocks, pal, mats, field = make_case(size, 31)
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    try:
+        p = radiance_params(rays=64, time_seed=2, alpha=0.1, max_steps=64)
+        best, ms = w.autotune(p, reps=2)
+        assert ms > 0
+        assert np.array_equal(w.read_radiance(), field)
+        assert w.get_tuning() == best
+        w.radiance_update(None, p)
+        torch.cuda.synchronize()
+        wd = orc.WorldData(blocks, full_palette(w, pal), mats)
+        ref = orc.radiance_update(wd, field, rays=64, time_seed=2, alpha=0.1, max_steps=64)
+        assert rel_err(w.read_radiance(), ref).max() <= REL
+    finally:
+        w.close()
+
+
+def test_radiance_after_map_uses_dynamic_slots(gpu_ctx):
+    """Radiance over a blockified world (models mapped) matches the oracle on
+    the same palette, for the tiled kernel with palette bits in LDS and in
+    global memory."""
+    import torch
+    size = (24, 24, 12)
+    blocks, pal, mats, field = make_case(size, 41)
+    rng = np.random.default_rng(42)
+    models, quats, shifts = synth.make_models(rng, n=6, lo=8, hi=32, world_size=size)
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    ms = [Model(gpu_ctx, m) for m in models]
+    try:
+        w.map_models(ms, quats, shifts)
+        torch.cuda.synchronize()
+        counter = w.palette_counter()
+        full = np.zeros((w.palette_slots, 16, 16, 16), np.uint8)
+        full[:counter] = w.get_palette(0, counter)
+        wd = orc.WorldData(w.get_blocks(True), full, mats)
+        ref = orc.radiance_update(wd, field, rays=64, time_seed=5, alpha=0.1)
+        for pb in (1, 0):
+            w.write_radiance(field)
+            w.set_tuning(**dict(TUNINGS["tiled"], lds_palette_bits=pb))
+            w.radiance_update(None, radiance_params(rays=64, time_seed=5, alpha=0.1))
+            torch.cuda.synchronize()
+            assert rel_err(w.read_radiance(), ref).max() <= REL
+    finally:
+        for m in ms:
+            m.close()
         w.close()
