@@ -89,6 +89,30 @@ def run_plain(args):
         return radiance_params(rays=rays, time_seed=frame, alpha=cfg["alpha"], max_dist=cfg["max_dist"],
                                max_steps=cfg["max_steps"])
 
+    if args.sweep:
+        sweep = []
+        for variant, threads, P, eb, pb in ([(0, 0, 0, 0, 0), (2, 0, 0, 0, 0)] +
+                                            [(1, th, P, eb, pb) for th in (256, 512, 1024) for P in (4, 16, 64)
+                                             for eb in (0, 1) for pb in (0, 1)]):
+            if P * rays > 4096:
+                continue
+            try:
+                w.set_tuning(variant=variant, threads=threads or 256, probes_per_batch=P or 16,
+                             lds_block_bits=eb, lds_palette_bits=pb, wg_per_cu=0)
+                w.radiance_update(None, params(0))
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for k in range(10):
+                    w.radiance_update(None, params(k))
+                e1.record()
+                torch.cuda.synchronize()
+                sweep.append(dict(variant=variant, threads=threads, P=P, eb=eb, pb=pb,
+                                  ms=e0.elapsed_time(e1) / 10))
+            except Exception as ex:  # shape does not fit
+                sweep.append(dict(variant=variant, threads=threads, P=P, eb=eb, pb=pb, error=str(ex)[:80]))
+        print(json.dumps({"sweep": sweep}), file=sys.stderr, flush=True)
+        w.write_radiance(field)
     tuning, tune_ms = w.autotune(params(0), reps=5)
     # algorithmic bytes of one launch (traced, untimed)
     keys, steps = w.radiance_update(None, params(0), traced=True)
@@ -133,6 +157,7 @@ def run_plain(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "radiance_tiled_kernel" if tuning["variant"] == 1 else "radiance_kernel",
+                     "variant": tuning["variant"],
                      "kernel_ms": kern_ms, "bytes_per_launch": bytes_per_launch},
         "tuning": tuning,
     }
@@ -153,17 +178,142 @@ def run_plain(args):
     return out
 
 
+def _time_frames(fn, n, torch):
+    """Device time (ms) of n calls of fn(k) issued on torch's current stream."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for k in range(n):
+        fn(k)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1), (time.perf_counter() - t0) * 1e3
+
+
+def run_batched(args):
+    """64 worlds (log-uniform 16x16x8..96x96x32 blocks, rays/probe from
+    {32,64,128}, 4 models each, 640x360 glossy each): the map -> radiance ->
+    glossy chain as one batched launch per pass captured in a hipGraph,
+    against plain one-world-per-launch passes."""
+    import torch
+    from lum_amd import workloads
+    from lum_amd.api import Context, Batch, Graph
+
+    ctx = Context(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    cases = workloads.batched_config(ctx, n_worlds=args.worlds)
+    worlds = [c.world for c in cases]
+    for c in cases:
+        c.world.set_tuning(variant=0)
+    batch = Batch(ctx, worlds)
+    rays_per_frame = sum(c.probe_rays for c in cases)
+    px_per_frame = sum(c.pixels for c in cases)
+    gmax = 128
+
+    def plain_frame(k):
+        for c in cases:
+            c.world.map_models_placed(c.models, c.placements)
+            c.params.time_seed = k
+            c.world.radiance_update(None, c.params)
+            c.world.glossy(c.gbuf[0], c.gbuf[1], c.gbuf[2], 128.0, gmax, out=c.out)
+
+    params = [c.params for c in cases]
+
+    def batched_pass():
+        for c in cases:
+            c.world.map_models_placed(c.models, c.placements)
+        batch.radiance(params)
+        batch.glossy([c.gbuf[0] for c in cases], [c.gbuf[1] for c in cases], [c.gbuf[2] for c in cases],
+                     [c.out for c in cases], 128.0, gmax)
+        batch.advance_frame()
+
+    def batched_frame(k):
+        batched_pass()
+
+    for c in cases:
+        c.params.time_seed = 0
+    for k in range(args.warmup):
+        plain_frame(k)
+        batched_frame(k)
+    frames = max(2, args.steps // 2 * 2)
+    plain_ms, plain_wall = _time_frames(plain_frame, frames, torch)
+    batched_ms, batched_wall = _time_frames(batched_frame, frames, torch)
+    # the same batched chain captured once (two frames: even/odd fields)
+    graph = Graph()
+    torch.cuda.synchronize()
+    graph.begin()
+    batched_pass()
+    batched_pass()
+    graph.end()
+    graph.launch()
+    torch.cuda.synchronize()
+    graph_ms, graph_wall = _time_frames(lambda k: graph.launch(), frames // 2, torch)
+    plain_graph = Graph()
+    plain_graph.begin()
+    plain_frame(0)
+    plain_frame(1)
+    plain_graph.end()
+    plain_graph.launch()
+    torch.cuda.synchronize()
+    pgraph_ms, pgraph_wall = _time_frames(lambda k: plain_graph.launch(), frames // 2, torch)
+    # dominant kernel: the batched radiance launch alone
+    def rad_only(k):
+        batch.radiance(params)
+    rad_ms, _ = _time_frames(rad_only, frames, torch)
+    graph.close()
+    plain_graph.close()
+    per_frame = {
+        "plain_eager_ms": plain_ms / frames,
+        "plain_graph_ms": pgraph_ms / (frames // 2) / 2,
+        "batched_eager_ms": batched_ms / frames,
+        "batched_graph_ms": graph_ms / (frames // 2) / 2,
+        "batched_radiance_ms": rad_ms / frames,
+    }
+    value = rays_per_frame / (per_frame["batched_graph_ms"] / 1e3)
+    plain_value = rays_per_frame / (per_frame["plain_eager_ms"] / 1e3)
+    out = {
+        "metric": "probe-rays/s, batched map->radiance->glossy chain (hipGraph) vs plain per-world launches",
+        "value": value,
+        "unit": "probe-rays/s",
+        "n_gpus": 1,
+        "steps": frames,
+        "warmup": args.warmup,
+        "ms_per_step": per_frame["batched_graph_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded worlds, palettes, materials, models, G-buffers)",
+        "config": {"workload": "batched: %d worlds log-uniform 16x16x8..96x96x32 blocks, rays/probe from "
+                               "{32,64,128}, 4 models 16^3-32^3 per world, glossy 640x360 per world, FP32"
+                               % len(cases),
+                   "probe_rays_per_frame": rays_per_frame, "glossy_pixels_per_frame": px_per_frame},
+        "plain_value": plain_value,
+        "speedup_vs_plain": value / plain_value,
+        "per_frame": per_frame,
+    }
+    batch.close()
+    for c in cases:
+        c.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "batched"])
+    ap.add_argument("--worlds", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--size", default="", help="override world size, e.g. 256,256,32")
     ap.add_argument("--rays", type=int, default=0)
     ap.add_argument("--max-steps", type=int, default=None)
+    ap.add_argument("--sweep", action="store_true", help="time every radiance launch shape (stderr)")
     ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
                     help="also run a plain device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -171,7 +321,7 @@ def main():
         print(f"bench.py measures one GPU only (the hot path has no multi-GPU form here); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
-    out = run_plain(args)
+    out = run_batched(args) if args.workload == "batched" else run_plain(args)
     print(json.dumps(out), flush=True)
 
 

@@ -63,6 +63,8 @@ extern "C" {
 typedef struct lumx_context* lumx_context_t;
 typedef struct lumx_world* lumx_world_t;
 typedef struct lumx_model* lumx_model_t;
+typedef struct lumx_batch* lumx_batch_t;
+typedef struct lumx_graph* lumx_graph_t;
 
 typedef struct {
     int size[3];          /* world size in 16^3 blocks (x, y, z) */
@@ -111,7 +113,8 @@ typedef struct {
 /* Launch shape of the radiance update.  No reference counterpart: it
  * replaces the fixed 64-lane workgroup of radiance.comp:37,360. */
 typedef struct {
-    int variant;            /* 0 = one thread per ray, a probe per wave; 1 = tiled persistent */
+    int variant;            /* 0 = one thread per ray, a probe per wave; 1 = tiled persistent;
+                               2 = as 0, marching the occupancy bits instead of the u8 palette */
     int threads;            /* workgroup size of the tiled kernel (multiple of 64, <= 1024) */
     int probes_per_batch;   /* probes per batch of the tiled kernel (power of two, 1..64) */
     int lds_block_bits;     /* stage the empty-block bits in LDS when they fit */
@@ -198,6 +201,32 @@ int lumx_glossy(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_d
                 int32_t* hit_keys_dev, void* stream);
 /* Lum's default camera (internal_types.hpp:54-66) for a frame size. */
 int lumx_camera_default(int frame_w, int frame_h, lumx_camera* out);
+
+/* ---- batches: many worlds per launch (no reference counterpart: the
+ * reference renders one world per frame) ---- */
+/* Worlds (distinct, one radiance format) updated together. */
+int lumx_batch_create(lumx_context_t ctx, int n, const lumx_world_t* worlds, lumx_batch_t* out);
+int lumx_batch_destroy(lumx_batch_t b);
+/* One radiance update of every world in one launch: requests_dev[i] /
+ * n_requests[i] as for lumx_radiance_update (requests_dev NULL = every probe
+ * of every world), params[i] per world (rays a power of two).  Each world's
+ * result equals its own lumx_radiance_update with the same arguments. */
+int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests_dev, const int* n_requests,
+                        const lumx_radiance_params* params, void* stream);
+/* Glossy pass of every world in one launch (one G-buffer and camera per world). */
+int lumx_batch_glossy(lumx_batch_t b, const uint8_t* const* matnorm_dev, const float* const* depth_dev,
+                      const int* width, const int* height, const lumx_camera* cams, float max_dist, int max_steps,
+                      float* const* out_dev, void* stream);
+/* Device frame counter added to every time_seed of this batch's radiance
+ * updates (0 at creation), so a replayed graph advances the ray rotation. */
+int lumx_batch_set_frame(lumx_batch_t b, int frame, void* stream);
+int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
+
+/* ---- graphs (hipGraph capture of any sequence of passes on a stream) ---- */
+int lumx_graph_begin(void* stream);
+int lumx_graph_end(void* stream, lumx_graph_t* out);
+int lumx_graph_launch(lumx_graph_t g, void* stream);
+int lumx_graph_destroy(lumx_graph_t g);
 
 /* ---- timing ---- */
 /* Time (ms) of the last pass of each kind on this world (0 radiance, 1 map, 2 glossy),

@@ -98,6 +98,17 @@ _SIGS = {
     "lumx_world_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "lumx_world_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "lumx_autotune_radiance": (_I, [_P, C.POINTER(RadianceParams), _I, C.POINTER(Tuning), C.POINTER(C.c_float)]),
+    "lumx_batch_create": (_I, [_P, _I, C.POINTER(_P), C.POINTER(_P)]),
+    "lumx_batch_destroy": (_I, [_P]),
+    "lumx_batch_radiance": (_I, [_P, C.POINTER(_P), C.POINTER(C.c_int), C.POINTER(RadianceParams), _P]),
+    "lumx_batch_glossy": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(Camera), C.c_float, _I, C.POINTER(_P), _P]),
+    "lumx_batch_set_frame": (_I, [_P, _I, _P]),
+    "lumx_batch_advance_frame": (_I, [_P, _P]),
+    "lumx_graph_begin": (_I, [_P]),
+    "lumx_graph_end": (_I, [_P, C.POINTER(_P)]),
+    "lumx_graph_launch": (_I, [_P, _P]),
+    "lumx_graph_destroy": (_I, [_P]),
 }
 
 _lib = None

@@ -217,3 +217,71 @@ class World:
         check(self.lib.lumx_glossy(self.handle, _ptr(matnorm), _ptr(depth), w, h, C.byref(cam),
                                    float(max_dist), int(max_steps), _ptr(out), _ptr(keys), _stream()))
         return (out, keys) if traced else out
+
+
+class Batch:
+    """Worlds updated together in one launch per pass (lumx_batch_*)."""
+
+    def __init__(self, ctx, worlds):
+        self.lib = ctx.lib
+        self.worlds = list(worlds)
+        hs = (C.c_void_p * len(self.worlds))(*[w.handle.value for w in self.worlds])
+        self.handle = C.c_void_p()
+        check(self.lib.lumx_batch_create(ctx.handle, len(self.worlds), hs, C.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            check(self.lib.lumx_batch_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def radiance(self, params, requests=None):
+        """params: one RadianceParams per world; requests: None or a list of
+        torch int32 (n, 4) device tensors (None entries = every probe)."""
+        n = len(self.worlds)
+        ps = (RadianceParams * n)(*params)
+        if requests is None:
+            check(self.lib.lumx_batch_radiance(self.handle, None, None, ps, _stream()))
+            return
+        rq = (C.c_void_p * n)(*[0 if r is None else r.data_ptr() for r in requests])
+        nr = (C.c_int * n)(*[0 if r is None else int(r.shape[0]) for r in requests])
+        check(self.lib.lumx_batch_radiance(self.handle, rq, nr, ps, _stream()))
+
+    def glossy(self, matnorms, depths, cams, outs, max_dist=128.0, max_steps=0):
+        n = len(self.worlds)
+        mn = (C.c_void_p * n)(*[m.data_ptr() for m in matnorms])
+        dp = (C.c_void_p * n)(*[d.data_ptr() for d in depths])
+        ws = (C.c_int * n)(*[int(d.shape[1]) for d in depths])
+        hs = (C.c_int * n)(*[int(d.shape[0]) for d in depths])
+        cs = (Camera * n)(*cams)
+        os_ = (C.c_void_p * n)(*[o.data_ptr() for o in outs])
+        check(self.lib.lumx_batch_glossy(self.handle, mn, dp, ws, hs, cs, float(max_dist), int(max_steps), os_,
+                                         _stream()))
+
+    def set_frame(self, frame):
+        check(self.lib.lumx_batch_set_frame(self.handle, int(frame), _stream()))
+
+    def advance_frame(self):
+        check(self.lib.lumx_batch_advance_frame(self.handle, _stream()))
+
+
+class Graph:
+    """hipGraph of the lumx passes issued on torch's current (non-null) stream
+    between begin() and end()."""
+
+    def __init__(self):
+        self.lib = load()
+        self.handle = C.c_void_p()
+
+    def begin(self):
+        check(self.lib.lumx_graph_begin(_stream()))
+
+    def end(self):
+        check(self.lib.lumx_graph_end(_stream(), C.byref(self.handle)))
+
+    def launch(self):
+        check(self.lib.lumx_graph_launch(self.handle, _stream()))
+
+    def close(self):
+        if self.handle:
+            check(self.lib.lumx_graph_destroy(self.handle))
+            self.handle = C.c_void_p()

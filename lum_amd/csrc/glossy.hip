@@ -11,11 +11,8 @@
 namespace lumx {
 
 template <int STORE, bool TRACED>
-__global__ void __launch_bounds__(256) glossy_kernel(GlossyArgs a) {
-    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (px >= a.width || py >= a.height) return;
-    const size_t pi = (size_t)py * a.width + px;
+__device__ __forceinline__ void glossy_pixel(const GlossyArgs& a, const int px, const int py) {
+    const unsigned pi = (unsigned)py * (unsigned)a.width + (unsigned)px;
     const WorldView& w = a.world;
     const uchar4 mn = a.matnorm[pi];
     const MatF mat = get_mat(w, mn.x);
@@ -61,6 +58,51 @@ __global__ void __launch_bounds__(256) glossy_kernel(GlossyArgs a) {
         L = L + sky * R;
     }
     a.out[pi] = make_float4(L.x / 8.0f, L.y / 8.0f, L.z / 8.0f, 1.0f - mat.rough);
+}
+
+template <int STORE, bool TRACED>
+__global__ void __launch_bounds__(256) glossy_kernel(GlossyArgs a) {
+    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (px >= a.width || py >= a.height) return;
+    glossy_pixel<STORE, TRACED>(a, px, py);
+}
+
+// Many frames in one launch: frame w owns threads [begin[w], begin[w] + w*h),
+// each range starting on a workgroup boundary, pixels row-major.
+template <int STORE>
+__global__ void __launch_bounds__(256) glossy_batch_kernel(const GlossyArgs* args, const int* begin, int n) {
+    const int b0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x));
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= b0) lo = mid;
+        else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const GlossyArgs a = args[lo];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - begin[lo];
+    if (i >= a.width * a.height) return;
+    const int py = i / a.width;
+    glossy_pixel<STORE, false>(a, i - py * a.width, py);
+}
+
+hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_threads, int store,
+                               hipStream_t stream) {
+    if (n <= 0 || total_threads <= 0) return hipSuccess;
+    const int grid = (total_threads + 255) / 256;
+    switch (store) {
+        case STORE_F16:
+            hipLaunchKernelGGL(glossy_batch_kernel<STORE_F16>, dim3(grid), dim3(256), 0, stream, args_dev, begin_dev, n);
+            break;
+        case STORE_BF16:
+            hipLaunchKernelGGL(glossy_batch_kernel<STORE_BF16>, dim3(grid), dim3(256), 0, stream, args_dev, begin_dev, n);
+            break;
+        default:
+            hipLaunchKernelGGL(glossy_batch_kernel<STORE_F32>, dim3(grid), dim3(256), 0, stream, args_dev, begin_dev, n);
+            break;
+    }
+    return hipGetLastError();
 }
 
 template <int STORE, bool TRACED>

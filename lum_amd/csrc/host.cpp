@@ -21,8 +21,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/lumx.h"
-#include "lumx_internal.hpp"
+#include "host_internal.hpp"
 
 using namespace lumx;
 
@@ -31,7 +30,7 @@ using namespace lumx;
 // ---------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
-static int fail(int code, const char* fmt, ...) {
+int lumx::fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -41,62 +40,19 @@ static int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                         \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail(LUMX_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                  \
-    } while (0)
+void lumx::clear_error() { g_last_error.clear(); }
 
-// ---------------------------------------------------------------------------
-// objects
-// ---------------------------------------------------------------------------
-struct lumx_context {
-    int device = 0;
-    int n_cu = 256;
-    hipStream_t stream = nullptr;
-    std::map<int, float4*> fib_tables;  // rays -> device table of per-lane directions
-    std::mutex mu;
-};
+bool lumx::stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+    return st == hipStreamCaptureStatusActive;
+}
 
-struct lumx_model {
-    lumx_context* ctx = nullptr;
-    int size[3] = {0, 0, 0};
-    uint8_t* voxels = nullptr;  // device, x-fastest
-};
-
-struct lumx_world {
-    lumx_context* ctx = nullptr;
-    lumx_world_desc desc{};
-    int n_blocks = 0;
-    std::vector<int16_t> origin;   // CPU origin world (Lum's origin_world)
-    std::vector<int16_t> current;  // blockified world of the last map
-    std::vector<int32_t> copy_src; // per palette slot: static source block
-    int counter = 0;               // palette slots in use (paletteCounter)
-    bool block0_empty = true;
-    std::vector<uint8_t> static_palette;  // host copy of the static slots
-    int16_t* blocks = nullptr;     // device blockified world
-    uint8_t* palette = nullptr;    // device palette slots
-    float* mats = nullptr;         // device materials
-    int32_t* copy_src_dev = nullptr;
-    void* field[2] = {nullptr, nullptr};
-    int cur = 0;
-    size_t texel_bytes = 16;
-    hipEvent_t ev[3][2] = {};
-    bool ev_valid[3] = {false, false, false};
-    int prev_counter = 0;               // palette counter of the previous map
-    uint32_t* empty_bits = nullptr;     // device: per block, 1 = id 0
-    uint32_t* pal_bits = nullptr;       // device: per slot, 128 words of occupancy bits
-    unsigned int* batch_counter = nullptr;  // device: tiled kernel work counter (2 words)
-    lumx_radiance_tuning tune{};
-};
-
-static const size_t kLdsBudget = 64 * 1024;
+static const size_t kLdsBudget = 60 * 1024;  // dynamic LDS per tiled workgroup
 
 static lumx_radiance_tuning default_tuning() {
     lumx_radiance_tuning t{};
-    t.variant = 1;
+    t.variant = 0;
     t.threads = 256;
     t.probes_per_batch = 16;
     t.lds_block_bits = 1;
@@ -239,7 +195,7 @@ static int get_fib(lumx_context* c, int rays, const float4** out) {
     return LUMX_OK;
 }
 
-static WorldView view_of(const lumx_world* w) {
+WorldView lumx::view_of(const lumx_world* w) {
     WorldView v;
     v.blocks = w->blocks;
     v.palette = w->palette;
@@ -275,6 +231,16 @@ int lumx_context_create(int device, lumx_context_t* out) {
         delete c;
         return fail(LUMX_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    // Direction tables of every power-of-two ray count up front, so that no
+    // pass allocates while a graph is being captured.
+    for (int r = 1; r <= 1024; r <<= 1) {
+        const float4* t = nullptr;
+        int rc = get_fib(c, r, &t);
+        if (rc) {
+            lumx_context_destroy(c);
+            return rc;
+        }
+    }
     *out = c;
     return LUMX_OK;
 }
@@ -309,6 +275,9 @@ static void world_free(lumx_world* w) {
     hipFree(w->empty_bits);
     hipFree(w->pal_bits);
     hipFree(w->batch_counter);
+    if (w->stage_blocks) hipHostFree(w->stage_blocks);
+    if (w->stage_done) hipEventDestroy(w->stage_done);
+    if (w->stage_copy_src) hipHostFree(w->stage_copy_src);
     for (int p = 0; p < 3; ++p)
         for (int k = 0; k < 2; ++k)
             if (w->ev[p][k]) hipEventDestroy(w->ev[p][k]);
@@ -355,6 +324,12 @@ int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* 
     chk(hipMalloc(&w->empty_bits, ((nb + 31) / 32) * sizeof(uint32_t)));
     chk(hipMalloc(&w->pal_bits, (size_t)slots * 128 * sizeof(uint32_t)));
     chk(hipMalloc(&w->batch_counter, 2 * sizeof(unsigned int)));
+    chk(hipHostMalloc((void**)&w->stage_blocks, nb * sizeof(int16_t), hipHostMallocDefault));
+    chk(hipEventCreateWithFlags(&w->stage_done, hipEventDisableTiming));
+    chk(hipHostMalloc((void**)&w->stage_copy_src, (size_t)slots * sizeof(int32_t), hipHostMallocDefault));
+    w->device_bytes = nb * sizeof(int16_t) + (size_t)slots * 4096 + 256 * 6 * sizeof(float) +
+                      (size_t)slots * sizeof(int32_t) + 2 * nb * w->texel_bytes + ((nb + 31) / 32) * 4 +
+                      (size_t)slots * 128 * 4 + 8;
     if (e == hipSuccess) {
         chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
         chk(hipMemset(w->pal_bits, 0, (size_t)slots * 128 * sizeof(uint32_t)));
@@ -558,13 +533,19 @@ int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes) {
 // the previous frame allocated beyond the new counter, so every slot at or
 // above the counter reads as air like the reference's cleared palette.
 static int commit_blocks(lumx_world* w, hipStream_t s) {
-    HIP_TRY(hipMemcpyAsync(w->blocks, w->current.data(), (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
+    // Pinned staging: the copies read it when they execute (also on graph
+    // replay), so wait for the previous map's copies before rewriting it.
+    if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
+    w->stage_pending = false;
+    std::memcpy(w->stage_blocks, w->current.data(), (size_t)w->n_blocks * 2);
+    HIP_TRY(hipMemcpyAsync(w->blocks, w->stage_blocks, (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_block_bits(w->blocks, w->n_blocks, w->empty_bits, s));
     const int st = w->desc.static_blocks;
     const int hi = std::max(w->counter, w->prev_counter);
     for (int k = w->counter; k < hi; ++k) w->copy_src[k] = 0;
     if (hi > st) {
-        HIP_TRY(hipMemcpyAsync(w->copy_src_dev + st, w->copy_src.data() + st, (size_t)(hi - st) * 4,
+        std::memcpy(w->stage_copy_src + st, w->copy_src.data() + st, (size_t)(hi - st) * 4);
+        HIP_TRY(hipMemcpyAsync(w->copy_src_dev + st, w->stage_copy_src + st, (size_t)(hi - st) * 4,
                                hipMemcpyHostToDevice, s));
         CopyArgs ca{w->palette, w->copy_src_dev, st, hi - st};
         HIP_TRY(launch_exec_copies(ca, s));
@@ -591,7 +572,9 @@ int lumx_world_upload_blocks(lumx_world_t w, void* stream) {
     if (rc) return rc;
     rc = commit_palette_bits(w, s);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(s));
+    if (stream_capturing(s)) return LUMX_OK;
+    HIP_TRY(hipEventRecord(w->stage_done, s));
+    w->stage_pending = true;
     return LUMX_OK;
 }
 
@@ -730,7 +713,8 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, co
         }
     }
     // end_blockify + exec_copies + map.
-    HIP_TRY(hipEventRecord(w->ev[1][0], s));
+    const bool capturing = stream_capturing(s);
+    if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));
     int rc = commit_blocks(w, s);
     if (rc) return rc;
     const int st = w->desc.static_blocks;
@@ -752,11 +736,11 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, co
     }
     rc = commit_palette_bits(w, s);
     if (rc) return rc;
+    if (capturing) return LUMX_OK;  // a graph re-reads the pinned staging on replay
     HIP_TRY(hipEventRecord(w->ev[1][1], s));
     w->ev_valid[1] = true;
-    // The host blockified world buffer is re-written by the next map: wait
-    // for the upload to finish before returning.
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipEventRecord(w->stage_done, s));
+    w->stage_pending = true;
     return LUMX_OK;
 }
 
@@ -772,7 +756,8 @@ int lumx_map_models(lumx_world_t w, int n, const lumx_model_t* models, const lum
 }
 
 // ---- radiance ----
-static int radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p) {
+extern "C++" {
+int lumx::radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p) {
     if (!w || !p) return fail(LUMX_EINVAL, "null argument");
     if (!valid_rays(p->rays))
         return fail(LUMX_EINVAL, "rays=%d: must be a power of two <= 64 or a multiple of 64 <= 1024", p->rays);
@@ -783,10 +768,11 @@ static int radiance_check(const lumx_world* w, const int32_t* req, int n_req, co
     if (req && n_req < 0) return fail(LUMX_EINVAL, "n_requests=%d invalid", n_req);
     return LUMX_OK;
 }
+}  // extern "C++"
 
 static int check_tuning(const lumx_world* w, const lumx_radiance_tuning& t, int rays) {
-    if (t.variant != 0 && t.variant != 1) return fail(LUMX_EINVAL, "tuning variant %d invalid", t.variant);
-    if (t.variant == 0) return LUMX_OK;
+    if (t.variant < 0 || t.variant > 2) return fail(LUMX_EINVAL, "tuning variant %d invalid", t.variant);
+    if (t.variant != 1) return LUMX_OK;
     if (t.threads < 64 || t.threads > 1024 || t.threads % 64)
         return fail(LUMX_EINVAL, "tuning threads %d invalid", t.threads);
     int P = t.probes_per_batch;
@@ -797,22 +783,23 @@ static int check_tuning(const lumx_world* w, const lumx_radiance_tuning& t, int 
     return LUMX_OK;
 }
 
-// Launch one radiance update from field[cur] into field[cur ^ 1] with tuning t.
-static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
-                           int32_t* keys, int32_t* steps, bool traced, hipStream_t s,
-                           const lumx_radiance_tuning& tune) {
+extern "C++" {
+int lumx::radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
+                        RadianceArgs* out) {
     int n_total = req ? n_req : w->n_blocks;
     int n_probes = n_total / p->size;
     const float4* fib = nullptr;
     int rc = get_fib(w->ctx, p->rays, &fib);
     if (rc) return rc;
-    RadianceArgs a{};
+    RadianceArgs& a = *out;
+    a = RadianceArgs{};
     a.world = view_of(w);
     a.field_in = w->field[w->cur];
     a.field_out = w->field[w->cur ^ 1];
     a.requests = (const int4*)req;
     a.fib = fib;
     a.rays = p->rays;
+    a.log2_rays = (p->rays & (p->rays - 1)) == 0 ? __builtin_ctz((unsigned)p->rays) : -1;
     a.n_probes = n_probes;
     a.time_seed = p->time_seed;
     a.size = p->size;
@@ -820,20 +807,31 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
     a.alpha = p->alpha;
     a.max_dist = p->max_dist;
     a.max_steps = p->max_steps;
-    a.ray_keys = keys;
-    a.ray_steps = steps;
     a.empty_bits = w->empty_bits;
     a.pal_bits = w->pal_bits;
     a.eb_words = (w->n_blocks + 31) / 32;
     a.pb_words = w->desc.palette_slots * 128;
     a.batch_counter = w->batch_counter;
+    return LUMX_OK;
+}
+}  // extern "C++"
+
+// Launch one radiance update from field[cur] into field[cur ^ 1] with tuning t.
+static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
+                           int32_t* keys, int32_t* steps, bool traced, hipStream_t s,
+                           const lumx_radiance_tuning& tune) {
+    RadianceArgs a;
+    int rc = radiance_args(w, req, n_req, p, &a);
+    if (rc) return rc;
+    a.ray_keys = keys;
+    a.ray_steps = steps;
     // Probes outside the request list keep their value in the new field.
     bool full = (req == nullptr && p->size == 1);
     if (!full)
         HIP_TRY(hipMemcpyAsync(a.field_out, a.field_in, (size_t)w->n_blocks * w->texel_bytes,
                                hipMemcpyDeviceToDevice, s));
-    if (tune.variant == 0) {
-        HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, s));
+    if (tune.variant == 0 || tune.variant == 2) {
+        HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, tune.variant == 2, s));
         return LUMX_OK;
     }
     RadianceTune t;
@@ -864,11 +862,14 @@ static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lum
     if (rc) return rc;
     HIP_TRY(hipSetDevice(w->ctx->device));
     hipStream_t s = pick_stream(w->ctx, stream);
-    HIP_TRY(hipEventRecord(w->ev[0][0], s));
+    const bool timed = !stream_capturing(s);
+    if (timed) HIP_TRY(hipEventRecord(w->ev[0][0], s));
     rc = radiance_launch(w, req, n_req, p, keys, steps, traced, s, w->tune);
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(w->ev[0][1], s));
-    w->ev_valid[0] = true;
+    if (timed) {
+        HIP_TRY(hipEventRecord(w->ev[0][1], s));
+        w->ev_valid[0] = true;
+    }
     w->cur ^= 1;
     return LUMX_OK;
 }
@@ -897,6 +898,8 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
     std::vector<lumx_radiance_tuning> cands;
     lumx_radiance_tuning c0{};
     c0.variant = 0;
+    cands.push_back(c0);
+    c0.variant = 2;
     cands.push_back(c0);
     const int threads[] = {256, 512, 1024};
     const int probes[] = {4, 8, 16, 32};
@@ -942,7 +945,7 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
     w->tune = best_c;
     if (best) *best = best_c;
     if (best_ms) *best_ms = best_t;
-    g_last_error.clear();
+    clear_error();
     return LUMX_OK;
 }
 
@@ -998,16 +1001,12 @@ int lumx_select_probes(lumx_world_t w, const int32_t* special, int n_special, in
 }
 
 // ---- glossy ----
-int lumx_glossy(lumx_world_t w, const uint8_t* matnorm, const float* depth, int width, int height,
-                const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys, void* stream) {
-    if (!w || !matnorm || !depth || !cam || !out) return fail(LUMX_EINVAL, "null argument");
-    if (width <= 0 || height <= 0 || width > 16384 || height > 16384)
-        return fail(LUMX_EINVAL, "frame %dx%d invalid", width, height);
-    if (!(max_dist > 0.0f && max_dist <= 65536.0f) || max_steps < 0)
-        return fail(LUMX_EINVAL, "max_dist must be in (0, 65536] and max_steps >= 0");
-    HIP_TRY(hipSetDevice(w->ctx->device));
-    hipStream_t s = pick_stream(w->ctx, stream);
-    GlossyArgs a;
+extern "C++" {
+void lumx::glossy_args(lumx_world* w, const uint8_t* matnorm, const float* depth, int width, int height,
+                       const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys,
+                       GlossyArgs* ap) {
+    GlossyArgs& a = *ap;
+    a = GlossyArgs{};
     a.world = view_of(w);
     a.field = w->field[w->cur];
     a.matnorm = (const uchar4*)matnorm;
@@ -1027,10 +1026,27 @@ int lumx_glossy(lumx_world_t w, const uint8_t* matnorm, const float* depth, int 
     a.frame_h = cam->frame_size[1];
     a.max_dist = max_dist;
     a.max_steps = max_steps;
-    HIP_TRY(hipEventRecord(w->ev[2][0], s));
+}
+}  // extern "C++"
+
+int lumx_glossy(lumx_world_t w, const uint8_t* matnorm, const float* depth, int width, int height,
+                const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys, void* stream) {
+    if (!w || !matnorm || !depth || !cam || !out) return fail(LUMX_EINVAL, "null argument");
+    if (width <= 0 || height <= 0 || width > 16384 || height > 16384)
+        return fail(LUMX_EINVAL, "frame %dx%d invalid", width, height);
+    if (!(max_dist > 0.0f && max_dist <= 65536.0f) || max_steps < 0)
+        return fail(LUMX_EINVAL, "max_dist must be in (0, 65536] and max_steps >= 0");
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    GlossyArgs a;
+    glossy_args(w, matnorm, depth, width, height, cam, max_dist, max_steps, out, keys, &a);
+    const bool timed = !stream_capturing(s);
+    if (timed) HIP_TRY(hipEventRecord(w->ev[2][0], s));
     HIP_TRY(launch_glossy(a, w->desc.radiance_format, keys != nullptr, s));
-    HIP_TRY(hipEventRecord(w->ev[2][1], s));
-    w->ev_valid[2] = true;
+    if (timed) {
+        HIP_TRY(hipEventRecord(w->ev[2][1], s));
+        w->ev_valid[2] = true;
+    }
     return LUMX_OK;
 }
 

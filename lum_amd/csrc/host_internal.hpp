@@ -1,0 +1,88 @@
+// host_internal.hpp - host-side objects of liblumx shared by host.cpp and
+// batch.cpp: contexts, worlds, models, error reporting.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lumx.h"
+#include "lumx_internal.hpp"
+
+namespace lumx {
+
+// Records the message of the calling thread's last failure; returns code.
+int fail(int code, const char* fmt, ...);
+void clear_error();
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::lumx::fail(LUMX_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                __FILE__, __LINE__);                                               \
+    } while (0)
+
+}  // namespace lumx
+
+struct lumx_context {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    std::map<int, float4*> fib_tables;  // rays -> device table of per-lane directions
+    std::mutex mu;
+};
+
+struct lumx_model {
+    lumx_context* ctx = nullptr;
+    int size[3] = {0, 0, 0};
+    uint8_t* voxels = nullptr;  // device, x-fastest
+};
+
+struct lumx_world {
+    lumx_context* ctx = nullptr;
+    lumx_world_desc desc{};
+    int n_blocks = 0;
+    std::vector<int16_t> origin;   // CPU origin world (Lum's origin_world)
+    std::vector<int16_t> current;  // blockified world of the last map
+    std::vector<int32_t> copy_src; // per palette slot: static source block
+    int counter = 0;               // palette slots in use (paletteCounter)
+    bool block0_empty = true;
+    std::vector<uint8_t> static_palette;  // host copy of the static slots
+    int16_t* blocks = nullptr;     // device blockified world
+    uint8_t* palette = nullptr;    // device palette slots
+    float* mats = nullptr;         // device materials
+    int32_t* copy_src_dev = nullptr;
+    void* field[2] = {nullptr, nullptr};
+    int cur = 0;
+    size_t texel_bytes = 16;
+    hipEvent_t ev[3][2] = {};
+    bool ev_valid[3] = {false, false, false};
+    int prev_counter = 0;               // palette counter of the previous map
+    uint32_t* empty_bits = nullptr;     // device: per block, 1 = id 0
+    uint32_t* pal_bits = nullptr;       // device: per slot, 128 words of occupancy bits
+    unsigned int* batch_counter = nullptr;  // device: tiled kernel work counter (2 words)
+    int16_t* stage_blocks = nullptr;    // pinned host staging of the blockified world
+    hipEvent_t stage_done = nullptr;    // recorded after the staging uploads of the last map
+    bool stage_pending = false;
+    int32_t* stage_copy_src = nullptr;  // pinned host staging of the copy list
+    lumx_radiance_tuning tune{};
+    size_t device_bytes = 0;            // HBM held by this world
+};
+
+namespace lumx {
+
+// Validation shared by single-world and batched radiance updates.
+int radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p);
+// Kernel arguments of one world's radiance update (field[cur] -> field[cur ^ 1]).
+int radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p, RadianceArgs* a);
+// Kernel arguments of one glossy pass over a G-buffer.
+void glossy_args(lumx_world* w, const uint8_t* matnorm, const float* depth, int width, int height,
+                 const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys, GlossyArgs* a);
+WorldView view_of(const lumx_world* w);
+bool stream_capturing(hipStream_t s);
+
+}  // namespace lumx

@@ -16,6 +16,7 @@ struct RadianceArgs {
     const int4* requests;   // probe list, or nullptr = every probe x-fastest
     const float4* fib;      // per-lane Fibonacci directions (xyz)
     int rays;
+    int log2_rays;          // log2(rays) when rays is a power of two, else -1
     int n_probes;           // dispatched probes (= n_requests / size)
     int time_seed;
     int size;
@@ -78,7 +79,7 @@ struct GlossyArgs {
     int max_steps;
 };
 
-hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, hipStream_t stream);
+hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, hipStream_t stream);
 hipError_t launch_radiance_tiled(RadianceArgs a, int store, bool traced, const RadianceTune& t, int n_cu,
                                  hipStream_t stream);
 size_t radiance_tiled_lds_bytes(const RadianceArgs& a, const RadianceTune& t);
@@ -88,5 +89,10 @@ hipError_t launch_palette_bits(const uint8_t* palette, int first_slot, int n_slo
 hipError_t launch_map(const MapArgs& a, hipStream_t stream);
 hipError_t launch_exec_copies(const CopyArgs& a, hipStream_t stream);
 hipError_t launch_glossy(const GlossyArgs& a, int store, bool traced, hipStream_t stream);
+hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
+                                 int total_threads, int block, int store, bool occ, const int* frame_dev,
+                                 hipStream_t stream);
+hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_threads, int store,
+                               hipStream_t stream);
 
 }  // namespace lumx

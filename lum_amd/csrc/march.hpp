@@ -66,22 +66,22 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 template <int STORE> struct Texel;
 template <> struct Texel<STORE_F32> {
     typedef float4 type;
-    __device__ __forceinline__ static f3 load(const void* base, size_t i) {
+    __device__ __forceinline__ static f3 load(const void* base, unsigned i) {
         float4 v = ((const float4*)base)[i];
         return {v.x, v.y, v.z};
     }
-    __device__ __forceinline__ static void store(void* base, size_t i, f3 v) {
+    __device__ __forceinline__ static void store(void* base, unsigned i, f3 v) {
         ((float4*)base)[i] = make_float4(v.x, v.y, v.z, 1.0f);
     }
 };
 template <> struct Texel<STORE_F16> {
-    __device__ __forceinline__ static f3 load(const void* base, size_t i) {
+    __device__ __forceinline__ static f3 load(const void* base, unsigned i) {
         uint2 raw = ((const uint2*)base)[i];
         _Float16 h[4];
         __builtin_memcpy(h, &raw, 8);
         return {(float)h[0], (float)h[1], (float)h[2]};
     }
-    __device__ __forceinline__ static void store(void* base, size_t i, f3 v) {
+    __device__ __forceinline__ static void store(void* base, unsigned i, f3 v) {
         _Float16 h[4] = {(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)1.0f};
         uint2 raw;
         __builtin_memcpy(&raw, h, 8);
@@ -89,12 +89,12 @@ template <> struct Texel<STORE_F16> {
     }
 };
 template <> struct Texel<STORE_BF16> {
-    __device__ __forceinline__ static f3 load(const void* base, size_t i) {
+    __device__ __forceinline__ static f3 load(const void* base, unsigned i) {
         uint2 raw = ((const uint2*)base)[i];
         return {bf16_to_f32((uint16_t)(raw.x & 0xffffu)), bf16_to_f32((uint16_t)(raw.x >> 16)),
                 bf16_to_f32((uint16_t)(raw.y & 0xffffu))};
     }
-    __device__ __forceinline__ static void store(void* base, size_t i, f3 v) {
+    __device__ __forceinline__ static void store(void* base, unsigned i, f3 v) {
         uint2 raw;
         raw.x = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
         raw.y = (uint32_t)f32_to_bf16(v.z) | ((uint32_t)0x3f80u << 16);
@@ -107,7 +107,7 @@ __device__ __forceinline__ int get_voxel(const WorldView& w, int ix, int iy, int
     int bx = ix / 16, by = iy / 16, bz = iz / 16;
     int b = 0;
     if ((unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz)
-        b = w.blocks[bx + w.wx * (by + w.wy * bz)];
+        b = w.blocks[(unsigned)(bx + w.wx * (by + w.wy * bz))];
     *blk_out = b;
     if (b == 0 && w.block0_empty) return 0;
     if ((unsigned)b >= (unsigned)w.palette_slots) return 0;
@@ -128,9 +128,14 @@ __device__ __forceinline__ MatF get_mat(const WorldView& w, int v) {
     return r;
 }
 
+// i mod n in [0, n); one conditional add or subtract covers i in [-n, 2n).
 __device__ __forceinline__ int wrap_i(int i, int n) {
-    int r = i % n;
-    return r < 0 ? r + n : r;
+    int r = i < 0 ? i + n : (i >= n ? i - n : i);
+    if ((unsigned)r >= (unsigned)n) {
+        r = i % n;
+        r = r < 0 ? r + n : r;
+    }
+    return r;
 }
 
 // sample_radiance: trilinear at texel coordinate (x + n*16)/16, REPEAT wrap.
@@ -143,15 +148,18 @@ __device__ __forceinline__ f3 sample_radiance(const void* field, int wx, int wy,
     int x0 = wrap_i(ix, wx), x1 = wrap_i(ix + 1, wx);
     int y0 = wrap_i(iy, wy), y1 = wrap_i(iy + 1, wy);
     int z0 = wrap_i(iz, wz), z1 = wrap_i(iz + 1, wz);
-    size_t sy = (size_t)wx, sz = (size_t)wx * wy;
-    f3 t000 = Texel<STORE>::load(field, x0 + sy * y0 + sz * z0);
-    f3 t100 = Texel<STORE>::load(field, x1 + sy * y0 + sz * z0);
-    f3 t010 = Texel<STORE>::load(field, x0 + sy * y1 + sz * z0);
-    f3 t110 = Texel<STORE>::load(field, x1 + sy * y1 + sz * z0);
-    f3 t001 = Texel<STORE>::load(field, x0 + sy * y0 + sz * z1);
-    f3 t101 = Texel<STORE>::load(field, x1 + sy * y0 + sz * z1);
-    f3 t011 = Texel<STORE>::load(field, x0 + sy * y1 + sz * z1);
-    f3 t111 = Texel<STORE>::load(field, x1 + sy * y1 + sz * z1);
+    const unsigned r00 = (unsigned)wx * ((unsigned)y0 + (unsigned)wy * (unsigned)z0);
+    const unsigned r10 = (unsigned)wx * ((unsigned)y1 + (unsigned)wy * (unsigned)z0);
+    const unsigned r01 = (unsigned)wx * ((unsigned)y0 + (unsigned)wy * (unsigned)z1);
+    const unsigned r11 = (unsigned)wx * ((unsigned)y1 + (unsigned)wy * (unsigned)z1);
+    f3 t000 = Texel<STORE>::load(field, r00 + (unsigned)x0);
+    f3 t100 = Texel<STORE>::load(field, r00 + (unsigned)x1);
+    f3 t010 = Texel<STORE>::load(field, r10 + (unsigned)x0);
+    f3 t110 = Texel<STORE>::load(field, r10 + (unsigned)x1);
+    f3 t001 = Texel<STORE>::load(field, r01 + (unsigned)x0);
+    f3 t101 = Texel<STORE>::load(field, r01 + (unsigned)x1);
+    f3 t011 = Texel<STORE>::load(field, r11 + (unsigned)x0);
+    f3 t111 = Texel<STORE>::load(field, r11 + (unsigned)x1);
     float ux = 1.0f - tx, uy = 1.0f - ty, uz = 1.0f - tz;
     f3 c00 = t000 * ux + t100 * tx;
     f3 c10 = t010 * ux + t110 * tx;
@@ -220,10 +228,10 @@ __device__ __forceinline__ RayHit cast_ray(const WorldView& w, f3 o, f3 d, float
     int sx = d.x > 0.0f ? 1 : -1, sy = d.y > 0.0f ? 1 : -1, sz = d.z > 0.0f ? 1 : -1;
     f3 fl = {floorf(bh.x), floorf(bh.y), floorf(bh.z)};
     f3 c1 = {(fl.x - bh.x) / d.x, (fl.y - bh.y) / d.y, (fl.z - bh.z) / d.z};
-    f3 c2 = {(fl.x - bh.x) / d.x + 1.0f / d.x, (fl.y - bh.y) / d.y + 1.0f / d.y,
-             (fl.z - bh.z) / d.z + 1.0f / d.z};
+    f3 c2 = {(fl.x - bh.x) / d.x + inv.x, (fl.y - bh.y) / d.y + inv.y, (fl.z - bh.z) / d.z + inv.z};
     f3 tMax = {fmaxf(c1.x, c2.x), fmaxf(c1.y, c2.y), fmaxf(c1.z, c2.z)};
-    f3 tDelta = {1.0f / fabsf(d.x), 1.0f / fabsf(d.y), 1.0f / fabsf(d.z)};
+    // 1/|d| == |1/d| exactly (round-to-nearest is sign-symmetric).
+    f3 tDelta = {fabsf(inv.x), fabsf(inv.y), fabsf(inv.z)};
     int vx = (int)bh.x, vy = (int)bh.y, vz = (int)bh.z;
     f3 st = {0.0f, 0.0f, 0.0f};
     int blk;
@@ -274,7 +282,7 @@ struct OccView {
     __device__ __forceinline__ int block_at(int ix, int iy, int iz) const {
         int bx = ix / 16, by = iy / 16, bz = iz / 16;
         if ((unsigned)bx >= (unsigned)wx || (unsigned)by >= (unsigned)wy || (unsigned)bz >= (unsigned)wz) return 0;
-        int lin = bx + wx * (by + wy * bz);
+        unsigned lin = (unsigned)(bx + wx * (by + wy * bz));
         if ((eb[lin >> 5] >> (lin & 31)) & 1u) return 0;
         return ids[lin];
     }
@@ -336,10 +344,10 @@ __device__ __forceinline__ RayHit cast_ray_occ(const OccView& w, f3 o, f3 d, flo
     int sx = d.x > 0.0f ? 1 : -1, sy = d.y > 0.0f ? 1 : -1, sz = d.z > 0.0f ? 1 : -1;
     f3 fl = {floorf(bh.x), floorf(bh.y), floorf(bh.z)};
     f3 c1 = {(fl.x - bh.x) / d.x, (fl.y - bh.y) / d.y, (fl.z - bh.z) / d.z};
-    f3 c2 = {(fl.x - bh.x) / d.x + 1.0f / d.x, (fl.y - bh.y) / d.y + 1.0f / d.y,
-             (fl.z - bh.z) / d.z + 1.0f / d.z};
+    f3 c2 = {(fl.x - bh.x) / d.x + inv.x, (fl.y - bh.y) / d.y + inv.y, (fl.z - bh.z) / d.z + inv.z};
     f3 tMax = {fmaxf(c1.x, c2.x), fmaxf(c1.y, c2.y), fmaxf(c1.z, c2.z)};
-    f3 tDelta = {1.0f / fabsf(d.x), 1.0f / fabsf(d.y), 1.0f / fabsf(d.z)};
+    // 1/|d| == |1/d| exactly (round-to-nearest is sign-symmetric).
+    f3 tDelta = {fabsf(inv.x), fabsf(inv.y), fabsf(inv.z)};
     int vx = (int)bh.x, vy = (int)bh.y, vz = (int)bh.z;
     f3 st = {0.0f, 0.0f, 0.0f};
     int vb = w.block_at(vx, vy, vz);

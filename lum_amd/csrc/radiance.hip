@@ -30,12 +30,14 @@ __device__ __forceinline__ void hash33(uint32_t x, uint32_t y, uint32_t z, f3* o
     *out = {(float)x * inv, (float)y * inv, (float)z * inv};
 }
 
-template <int STORE, bool TRACED>
-__global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
-    extern __shared__ f3 chunk_sum[];
+// One ray per thread; tid is the thread's index within this world's launch
+// range, which starts on a workgroup boundary.
+template <int STORE, bool TRACED, bool OCC>
+__device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const int tid, f3* chunk_sum) {
     const int R = a.rays;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int wg = tid / R;
+    int wg = a.log2_rays >= 0 ? (tid >> a.log2_rays) : tid / R;
+    // With R >= 64 a wave holds rays of one probe: keep the probe on the scalar unit.
+    if (R >= 64) wg = __builtin_amdgcn_readfirstlane(wg);
     const int lane = tid - wg * R;
     const bool active = wg < a.n_probes;
 
@@ -71,7 +73,22 @@ __global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
         pos = pos + (r1 - half) * 2.0f;
         if (lane & 1) dir = -dir;
 
-        RayHit h = cast_ray(w, pos, dir, a.max_dist, a.max_steps);
+        RayHit h;
+        if (OCC) {
+            OccView occ;
+            occ.ids = w.blocks;
+            occ.palette = w.palette;
+            occ.eb = a.empty_bits;
+            occ.pb = a.pal_bits;
+            occ.wx = w.wx;
+            occ.wy = w.wy;
+            occ.wz = w.wz;
+            occ.palette_slots = w.palette_slots;
+            occ.block0_empty = w.block0_empty;
+            h = cast_ray_occ(occ, pos, dir, a.max_dist, a.max_steps);
+        } else {
+            h = cast_ray(w, pos, dir, a.max_dist, a.max_steps);
+        }
         if (h.hit) {
             MatF m = get_mat(w, h.voxel);
             f3 prop = sample_radiance<STORE>(a.field_in, w.wx, w.wy, w.wz, pos + h.fraction * dir, h.normal);
@@ -116,7 +133,7 @@ __global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
     }
     if (valid && lane == 0) {
         f3 mean = total / (float)R;
-        size_t ti = (size_t)px + (size_t)w.wx * ((size_t)py + (size_t)w.wy * pz);
+        const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
         f3 old = Texel<STORE>::load(a.field_in, ti);
         old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
         const float al = a.alpha, bl = 1.0f - a.alpha;
@@ -125,31 +142,84 @@ __global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
     }
 }
 
-template <int STORE, bool TRACED>
+template <int STORE, bool TRACED, bool OCC>
+__global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
+    extern __shared__ f3 chunk_sum[];
+    radiance_lane_body<STORE, TRACED, OCC>(a, blockIdx.x * blockDim.x + threadIdx.x, chunk_sum);
+}
+
+// Many worlds in one launch.  World w owns threads [begin[w], begin[w+1]),
+// each range starting on a workgroup boundary; a workgroup finds its world by
+// a scalar binary search over begin[].  The per-probe arithmetic, RNG keying
+// (request index within the world's own update) and reduction order are those
+// of a single-world launch, so every world's result is the same alone and in
+// the batch.  time_seed of every world is offset by *frame (0 when null).
+template <int STORE, bool OCC>
+__global__ void __launch_bounds__(1024)
+radiance_batch_kernel(const RadianceArgs* args, const int* begin, int n_worlds, const int* frame) {
+    extern __shared__ f3 chunk_sum[];
+    const int b0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x));
+    int lo = 0, hi = n_worlds - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= b0) lo = mid;
+        else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    RadianceArgs a = args[lo];
+    if (frame) a.time_seed += *frame;
+    radiance_lane_body<STORE, false, OCC>(a, (int)(blockIdx.x * blockDim.x + threadIdx.x) - begin[lo], chunk_sum);
+}
+
+hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
+                                 int total_threads, int block, int store, bool occ, const int* frame_dev,
+                                 hipStream_t stream) {
+    if (total_threads <= 0 || n_worlds <= 0) return hipSuccess;
+    const int grid = (total_threads + block - 1) / block;
+    const size_t lds = sizeof(f3) * (block / 64);
+#define LUMX_BATCH_LAUNCH(S, O) \
+    hipLaunchKernelGGL((radiance_batch_kernel<S, O>), dim3(grid), dim3(block), lds, stream, args_dev, begin_dev, \
+                       n_worlds, frame_dev)
+    switch (store) {
+        case STORE_F16: if (occ) LUMX_BATCH_LAUNCH(STORE_F16, true); else LUMX_BATCH_LAUNCH(STORE_F16, false); break;
+        case STORE_BF16: if (occ) LUMX_BATCH_LAUNCH(STORE_BF16, true); else LUMX_BATCH_LAUNCH(STORE_BF16, false); break;
+        default: if (occ) LUMX_BATCH_LAUNCH(STORE_F32, true); else LUMX_BATCH_LAUNCH(STORE_F32, false); break;
+    }
+#undef LUMX_BATCH_LAUNCH
+    return hipGetLastError();
+}
+
+template <int STORE, bool TRACED, bool OCC>
 static hipError_t launch_store(const RadianceArgs& a, hipStream_t stream) {
     const int R = a.rays;
     int bs = R >= 256 ? R : R * (256 / R);
     long long threads = (long long)a.n_probes * R;
     int grid = (int)((threads + bs - 1) / bs);
-    size_t lds = R > 64 ? sizeof(f3) * (bs / 64) : 0;
+    size_t lds = sizeof(f3) * (bs / 64);
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL((radiance_kernel<STORE, TRACED>), dim3(grid), dim3(bs), lds, stream, a);
+    hipLaunchKernelGGL((radiance_kernel<STORE, TRACED, OCC>), dim3(grid), dim3(bs), lds, stream, a);
     return hipGetLastError();
 }
 
-hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, hipStream_t stream) {
+template <bool OCC>
+static hipError_t launch_lane(const RadianceArgs& a, int store, bool traced, hipStream_t stream) {
     if (traced) {
         switch (store) {
-            case STORE_F16: return launch_store<STORE_F16, true>(a, stream);
-            case STORE_BF16: return launch_store<STORE_BF16, true>(a, stream);
-            default: return launch_store<STORE_F32, true>(a, stream);
+            case STORE_F16: return launch_store<STORE_F16, true, OCC>(a, stream);
+            case STORE_BF16: return launch_store<STORE_BF16, true, OCC>(a, stream);
+            default: return launch_store<STORE_F32, true, OCC>(a, stream);
         }
     }
     switch (store) {
-        case STORE_F16: return launch_store<STORE_F16, false>(a, stream);
-        case STORE_BF16: return launch_store<STORE_BF16, false>(a, stream);
-        default: return launch_store<STORE_F32, false>(a, stream);
+        case STORE_F16: return launch_store<STORE_F16, false, OCC>(a, stream);
+        case STORE_BF16: return launch_store<STORE_BF16, false, OCC>(a, stream);
+        default: return launch_store<STORE_F32, false, OCC>(a, stream);
     }
+}
+
+// occ: march over the occupancy bits (global memory) instead of the u8 palette.
+hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, hipStream_t stream) {
+    return occ ? launch_lane<true>(a, store, traced, stream) : launch_lane<false>(a, store, traced, stream);
 }
 
 }  // namespace lumx
@@ -319,7 +389,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
                 total = total + f3{s_lx[q], s_ly[q], s_lz[q]};
             }
             f3 mean = total / (float)R;
-            size_t ti = (size_t)px + (size_t)wv.wx * ((size_t)py + (size_t)wv.wy * pz);
+            const unsigned ti = (unsigned)px + (unsigned)wv.wx * ((unsigned)py + (unsigned)wv.wy * (unsigned)pz);
             f3 old = Texel<STORE>::load(a.field_in, ti);
             old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
             const float al = a.alpha, bl = 1.0f - a.alpha;

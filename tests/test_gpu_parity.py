@@ -51,6 +51,7 @@ def full_palette(world, pal):
 
 TUNINGS = {
     "lane": dict(variant=0),
+    "lane_occ": dict(variant=2),
     "tiled": dict(variant=1, threads=256, probes_per_batch=16, lds_block_bits=1, lds_palette_bits=1),
     "tiled_global_bits": dict(variant=1, threads=1024, probes_per_batch=4, lds_block_bits=0, lds_palette_bits=0),
     "tiled_p32": dict(variant=1, threads=512, probes_per_batch=32, lds_block_bits=1, lds_palette_bits=0),
