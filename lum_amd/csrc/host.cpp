@@ -277,6 +277,7 @@ static void world_free(lumx_world* w) {
     hipFree(w->batch_counter);
     if (w->stage_blocks) hipHostFree(w->stage_blocks);
     if (w->stage_done) hipEventDestroy(w->stage_done);
+    for (void* p : w->graph_stage) hipHostFree(p);
     if (w->stage_copy_src) hipHostFree(w->stage_copy_src);
     for (int p = 0; p < 3; ++p)
         for (int k = 0; k < 2; ++k)
@@ -533,19 +534,31 @@ int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes) {
 // the previous frame allocated beyond the new counter, so every slot at or
 // above the counter reads as air like the reference's cleared palette.
 static int commit_blocks(lumx_world* w, hipStream_t s) {
-    // Pinned staging: the copies read it when they execute (also on graph
-    // replay), so wait for the previous map's copies before rewriting it.
-    if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
-    w->stage_pending = false;
-    std::memcpy(w->stage_blocks, w->current.data(), (size_t)w->n_blocks * 2);
-    HIP_TRY(hipMemcpyAsync(w->blocks, w->stage_blocks, (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
+    // Pinned staging: the copies read it when they execute.  Eager maps share
+    // one staging and wait for the previous map's copies before rewriting it;
+    // a map recorded into a graph gets staging of its own, read on every replay.
+    int16_t* stage_blocks = w->stage_blocks;
+    int32_t* stage_copy = w->stage_copy_src;
+    if (stream_capturing(s)) {
+        void* p = nullptr;
+        const size_t bytes = (size_t)w->n_blocks * 2 + (size_t)w->desc.palette_slots * 4 + 16;
+        HIP_TRY(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        w->graph_stage.push_back(p);
+        stage_blocks = (int16_t*)p;
+        stage_copy = (int32_t*)((char*)p + (((size_t)w->n_blocks * 2 + 15) / 16) * 16);
+    } else {
+        if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
+        w->stage_pending = false;
+    }
+    std::memcpy(stage_blocks, w->current.data(), (size_t)w->n_blocks * 2);
+    HIP_TRY(hipMemcpyAsync(w->blocks, stage_blocks, (size_t)w->n_blocks * 2, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_block_bits(w->blocks, w->n_blocks, w->empty_bits, s));
     const int st = w->desc.static_blocks;
     const int hi = std::max(w->counter, w->prev_counter);
     for (int k = w->counter; k < hi; ++k) w->copy_src[k] = 0;
     if (hi > st) {
-        std::memcpy(w->stage_copy_src + st, w->copy_src.data() + st, (size_t)(hi - st) * 4);
-        HIP_TRY(hipMemcpyAsync(w->copy_src_dev + st, w->stage_copy_src + st, (size_t)(hi - st) * 4,
+        std::memcpy(stage_copy + st, w->copy_src.data() + st, (size_t)(hi - st) * 4);
+        HIP_TRY(hipMemcpyAsync(w->copy_src_dev + st, stage_copy + st, (size_t)(hi - st) * 4,
                                hipMemcpyHostToDevice, s));
         CopyArgs ca{w->palette, w->copy_src_dev, st, hi - st};
         HIP_TRY(launch_exec_copies(ca, s));

@@ -68,6 +68,7 @@ struct lumx_world {
     int16_t* stage_blocks = nullptr;    // pinned host staging of the blockified world
     hipEvent_t stage_done = nullptr;    // recorded after the staging uploads of the last map
     bool stage_pending = false;
+    std::vector<void*> graph_stage;     // pinned staging owned by maps recorded into graphs
     int32_t* stage_copy_src = nullptr;  // pinned host staging of the copy list
     lumx_radiance_tuning tune{};
     size_t device_bytes = 0;            // HBM held by this world
