@@ -152,6 +152,14 @@ int lumx_world_read_radiance(lumx_world_t w, float* rgba_host);
 int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes);
 /* Upload the origin world as the current world (no models mapped). */
 int lumx_world_upload_blocks(lumx_world_t w, void* stream);
+/* Residency: evict copies the world's device state (blockified world,
+ * palette slots in use, materials, current radiance field) to host memory and
+ * frees its HBM; restore re-creates it bit for bit.  Passes on an evicted
+ * world fail with LUMX_EINVAL.  No reference counterpart (the reference keeps
+ * one world resident); used by the serving engine's HBM budget. */
+int lumx_world_evict(lumx_world_t w);
+int lumx_world_restore(lumx_world_t w);
+int lumx_world_resident(lumx_world_t w, int* resident, size_t* device_bytes);
 
 /* ---- models ---- */
 int lumx_model_create(lumx_context_t ctx, const uint8_t* voxels_host, int sx, int sy, int sz,
@@ -221,6 +229,63 @@ int lumx_batch_glossy(lumx_batch_t b, const uint8_t* const* matnorm_dev, const f
  * updates (0 at creation), so a replayed graph advances the ray rotation. */
 int lumx_batch_set_frame(lumx_batch_t b, int frame, void* stream);
 int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
+
+/* ---- serving: continuous batching over resident worlds (no reference
+ * counterpart: the reference renders one world per frame) ----
+ * A worker thread admits submitted requests in arrival order, at most one
+ * per world per launch and in submission order per world, restoring evicted
+ * worlds (and evicting least-recently-used idle ones) to stay within the HBM
+ * budget, packs up to max_batch requests into one batched radiance launch
+ * plus one batched glossy launch, and keeps up to `streams` launches in
+ * flight while it admits the next.  Each request computes exactly what its
+ * own lumx_radiance_update / lumx_glossy call would.  Worlds added to a
+ * server belong to it until it is destroyed. */
+#define LUMX_REQ_RADIANCE 0
+#define LUMX_REQ_GLOSSY 1
+
+typedef struct lumx_server* lumx_server_t;
+
+typedef struct {
+    size_t hbm_budget;  /* bytes of resident world state, 0 = no limit */
+    int max_batch;      /* requests per launch; 1 = one request per launch */
+    int streams;        /* launches in flight at once (>= 1) */
+    int reserved;
+} lumx_server_desc;
+
+typedef struct {
+    int kind;                    /* LUMX_REQ_RADIANCE or LUMX_REQ_GLOSSY */
+    int world;                   /* id from lumx_server_add_world */
+    const int32_t* requests_dev; /* radiance: probe list (int32x4) or NULL = every probe */
+    int n_requests;
+    lumx_radiance_params params; /* radiance (rays a power of two) */
+    const uint8_t* matnorm_dev;  /* glossy: G-buffer, camera, output */
+    const float* depth_dev;
+    int width, height;
+    lumx_camera cam;
+    float max_dist;
+    int max_steps;
+    float* out_dev;
+} lumx_request;
+
+typedef struct {
+    long long submitted, completed, launches, probe_rays, pixels, evictions, restores;
+    size_t resident_bytes;
+} lumx_server_stats;
+
+int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_server_t* out);
+int lumx_server_destroy(lumx_server_t s);
+int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id);
+/* Thread-safe.  The ticket identifies the request in lumx_server_wait. */
+int lumx_server_submit(lumx_server_t s, const lumx_request* r, long long* ticket);
+/* Blocks until the request finished; latency = completion - submission. */
+int lumx_server_wait(lumx_server_t s, long long ticket, double* latency_s);
+int lumx_server_drain(lumx_server_t s);
+int lumx_server_get_stats(lumx_server_t s, lumx_server_stats* out);
+/* Open-loop load: submits reqs[i] at arrival_s[i] seconds after the call
+ * (from a submitter thread), waits for all of them and returns each latency
+ * and the wall time from the first arrival to the last completion. */
+int lumx_server_run_trace(lumx_server_t s, int n, const lumx_request* reqs, const double* arrival_s,
+                          double* latency_s, double* wall_s);
 
 /* ---- graphs (hipGraph capture of any sequence of passes on a stream) ---- */
 int lumx_graph_begin(void* stream);

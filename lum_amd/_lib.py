@@ -58,6 +58,26 @@ class Camera(C.Structure):
                 ("verti_scaled", C.c_float * 3), ("light_dir", C.c_float * 3), ("frame_size", C.c_float * 2)]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("streams", C.c_int), ("reserved", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("world", C.c_int), ("requests_dev", C.c_void_p), ("n_requests", C.c_int),
+                ("params", RadianceParams), ("matnorm_dev", C.c_void_p), ("depth_dev", C.c_void_p),
+                ("width", C.c_int), ("height", C.c_int), ("cam", Camera), ("max_dist", C.c_float),
+                ("max_steps", C.c_int), ("out_dev", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("launches", C.c_longlong),
+                ("probe_rays", C.c_longlong), ("pixels", C.c_longlong), ("evictions", C.c_longlong),
+                ("restores", C.c_longlong), ("resident_bytes", C.c_size_t)]
+
+
+REQ_RADIANCE = 0
+REQ_GLOSSY = 1
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _I = C.c_int
@@ -105,6 +125,18 @@ _SIGS = {
                                C.POINTER(Camera), C.c_float, _I, C.POINTER(_P), _P]),
     "lumx_batch_set_frame": (_I, [_P, _I, _P]),
     "lumx_batch_advance_frame": (_I, [_P, _P]),
+    "lumx_world_evict": (_I, [_P]),
+    "lumx_world_restore": (_I, [_P]),
+    "lumx_world_resident": (_I, [_P, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
+    "lumx_server_create": (_I, [_P, C.POINTER(ServerDesc), C.POINTER(_P)]),
+    "lumx_server_destroy": (_I, [_P]),
+    "lumx_server_add_world": (_I, [_P, _P, C.POINTER(C.c_int)]),
+    "lumx_server_submit": (_I, [_P, C.POINTER(Request), C.POINTER(C.c_longlong)]),
+    "lumx_server_wait": (_I, [_P, C.c_longlong, C.POINTER(C.c_double)]),
+    "lumx_server_drain": (_I, [_P]),
+    "lumx_server_get_stats": (_I, [_P, C.POINTER(ServerStats)]),
+    "lumx_server_run_trace": (_I, [_P, _I, C.POINTER(Request), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
     "lumx_graph_begin": (_I, [_P]),
     "lumx_graph_end": (_I, [_P, C.POINTER(_P)]),
     "lumx_graph_launch": (_I, [_P, _P]),

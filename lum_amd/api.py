@@ -10,7 +10,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (Camera, Placement, RadianceParams, Transform, Tuning, WorldDesc, check, load, RADIANCE_F32,
-                   PALETTE_XYZ_XFAST)
+                   PALETTE_XYZ_XFAST, ServerDesc, Request, ServerStats, REQ_RADIANCE, REQ_GLOSSY)
 
 TUNING_FIELDS = ("variant", "threads", "probes_per_batch", "lds_block_bits", "lds_palette_bits", "wg_per_cu")
 
@@ -170,6 +170,18 @@ class World:
         check(self.lib.lumx_autotune_radiance(self.handle, C.byref(params), reps, C.byref(t), C.byref(ms)))
         return {k: getattr(t, k) for k in TUNING_FIELDS}, ms.value
 
+    def evict(self):
+        check(self.lib.lumx_world_evict(self.handle))
+
+    def restore(self):
+        check(self.lib.lumx_world_restore(self.handle))
+
+    def resident(self):
+        r = C.c_int()
+        b = C.c_size_t()
+        check(self.lib.lumx_world_resident(self.handle, C.byref(r), C.byref(b)))
+        return bool(r.value), b.value
+
     # -- passes --
     def map_models(self, models, rots, shifts):
         n = len(models)
@@ -284,4 +296,75 @@ class Graph:
     def close(self):
         if self.handle:
             check(self.lib.lumx_graph_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+
+def radiance_request(world_id, params, requests=None):
+    r = Request()
+    r.kind = REQ_RADIANCE
+    r.world = world_id
+    r.requests_dev = 0 if requests is None else requests.data_ptr()
+    r.n_requests = 0 if requests is None else int(requests.shape[0])
+    r.params = params
+    return r
+
+
+def glossy_request(world_id, matnorm, depth, cam, out, max_dist=128.0, max_steps=0):
+    r = Request()
+    r.kind = REQ_GLOSSY
+    r.world = world_id
+    r.matnorm_dev = matnorm.data_ptr()
+    r.depth_dev = depth.data_ptr()
+    r.height, r.width = int(depth.shape[0]), int(depth.shape[1])
+    r.cam = cam
+    r.max_dist = float(max_dist)
+    r.max_steps = int(max_steps)
+    r.out_dev = out.data_ptr()
+    return r
+
+
+class Server:
+    """Continuous-batching request engine (lumx_server_*)."""
+
+    def __init__(self, ctx, worlds, hbm_budget=0, max_batch=32, streams=2):
+        self.lib = ctx.lib
+        d = ServerDesc(int(hbm_budget), int(max_batch), int(streams), 0)
+        self.handle = C.c_void_p()
+        check(self.lib.lumx_server_create(ctx.handle, C.byref(d), C.byref(self.handle)))
+        self.ids = []
+        for w in worlds:
+            i = C.c_int()
+            check(self.lib.lumx_server_add_world(self.handle, w.handle, C.byref(i)))
+            self.ids.append(i.value)
+
+    def submit(self, req):
+        t = C.c_longlong()
+        check(self.lib.lumx_server_submit(self.handle, C.byref(req), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        lat = C.c_double()
+        check(self.lib.lumx_server_wait(self.handle, int(ticket), C.byref(lat)))
+        return lat.value
+
+    def drain(self):
+        check(self.lib.lumx_server_drain(self.handle))
+
+    def stats(self):
+        st = ServerStats()
+        check(self.lib.lumx_server_get_stats(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def run_trace(self, reqs, arrivals):
+        n = len(reqs)
+        arr = (Request * max(n, 1))(*reqs)
+        at = (C.c_double * max(n, 1))(*arrivals)
+        lat = (C.c_double * max(n, 1))()
+        wall = C.c_double()
+        check(self.lib.lumx_server_run_trace(self.handle, n, arr, at, lat, C.byref(wall)))
+        return np.array(lat[:n]), wall.value
+
+    def close(self):
+        if self.handle:
+            check(self.lib.lumx_server_destroy(self.handle))
             self.handle = C.c_void_p()

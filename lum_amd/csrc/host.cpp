@@ -65,6 +65,11 @@ static lumx_radiance_tuning default_tuning() {
 // non-blocking stream is available through lumx_context_stream().
 static hipStream_t pick_stream(lumx_context*, void* s) { return (hipStream_t)s; }
 
+#define REQUIRE_RESIDENT(w) \
+    do { \
+        if (!(w)->blocks) return fail(LUMX_EINVAL, "world is evicted (lumx_world_restore first)"); \
+    } while (0)
+
 // ---------------------------------------------------------------------------
 // host math: quaternion -> matrix, AABB of get_shift, inverse, camera
 // ---------------------------------------------------------------------------
@@ -265,7 +270,41 @@ int lumx_context_sync(lumx_context_t c) {
 void* lumx_context_stream(lumx_context_t c) { return c ? (void*)c->stream : nullptr; }
 
 // ---- world ----
-static void world_free(lumx_world* w) {
+// Device state of a world: blocks, palette slots, materials, copy list,
+// two radiance fields, occupancy bits, work counter (all zeroed).
+static hipError_t world_alloc_device(lumx_world* w) {
+    const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    chk(hipMalloc(&w->blocks, nb * sizeof(int16_t)));
+    chk(hipMalloc(&w->palette, slots * 4096));
+    chk(hipMalloc(&w->mats, 256 * 6 * sizeof(float)));
+    chk(hipMalloc(&w->copy_src_dev, slots * sizeof(int32_t)));
+    chk(hipMalloc(&w->field[0], nb * w->texel_bytes));
+    chk(hipMalloc(&w->field[1], nb * w->texel_bytes));
+    chk(hipMalloc(&w->empty_bits, ((nb + 31) / 32) * sizeof(uint32_t)));
+    chk(hipMalloc(&w->pal_bits, slots * 128 * sizeof(uint32_t)));
+    chk(hipMalloc(&w->batch_counter, 2 * sizeof(unsigned int)));
+    if (e != hipSuccess) return e;
+    chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
+    chk(hipMemset(w->pal_bits, 0, slots * 128 * sizeof(uint32_t)));
+    chk(hipMemset(w->batch_counter, 0, 2 * sizeof(unsigned int)));
+    chk(hipMemset(w->blocks, 0, nb * sizeof(int16_t)));
+    chk(hipMemset(w->palette, 0, slots * 4096));
+    chk(hipMemset(w->mats, 0, 256 * 6 * sizeof(float)));
+    chk(hipMemset(w->copy_src_dev, 0, slots * sizeof(int32_t)));
+    chk(hipMemset(w->field[0], 0, nb * w->texel_bytes));
+    chk(hipMemset(w->field[1], 0, nb * w->texel_bytes));
+    return e;
+}
+
+static size_t world_device_bytes(const lumx_world* w) {
+    const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
+    return nb * 2 + slots * 4096 + 256 * 6 * 4 + slots * 4 + 2 * nb * w->texel_bytes + ((nb + 31) / 32) * 4 +
+           slots * 128 * 4 + 8;
+}
+
+static void world_free_device(lumx_world* w) {
     hipFree(w->blocks);
     hipFree(w->palette);
     hipFree(w->mats);
@@ -275,10 +314,22 @@ static void world_free(lumx_world* w) {
     hipFree(w->empty_bits);
     hipFree(w->pal_bits);
     hipFree(w->batch_counter);
+    w->blocks = nullptr;
+    w->palette = nullptr;
+    w->mats = nullptr;
+    w->copy_src_dev = nullptr;
+    w->field[0] = w->field[1] = nullptr;
+    w->empty_bits = nullptr;
+    w->pal_bits = nullptr;
+    w->batch_counter = nullptr;
+}
+
+static void world_free(lumx_world* w) {
+    world_free_device(w);
     if (w->stage_blocks) hipHostFree(w->stage_blocks);
+    if (w->stage_copy_src) hipHostFree(w->stage_copy_src);
     if (w->stage_done) hipEventDestroy(w->stage_done);
     for (void* p : w->graph_stage) hipHostFree(p);
-    if (w->stage_copy_src) hipHostFree(w->stage_copy_src);
     for (int p = 0; p < 3; ++p)
         for (int k = 0; k < 2; ++k)
             if (w->ev[p][k]) hipEventDestroy(w->ev[p][k]);
@@ -314,36 +365,14 @@ int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* 
     w->counter = d->static_blocks;
     w->prev_counter = d->static_blocks;
     w->tune = default_tuning();
-    hipError_t e = hipSuccess;
+    w->device_bytes = world_device_bytes(w);
+    hipError_t e = world_alloc_device(w);
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
-    chk(hipMalloc(&w->blocks, nb * sizeof(int16_t)));
-    chk(hipMalloc(&w->palette, (size_t)slots * 4096));
-    chk(hipMalloc(&w->mats, 256 * 6 * sizeof(float)));
-    chk(hipMalloc(&w->copy_src_dev, (size_t)slots * sizeof(int32_t)));
-    chk(hipMalloc(&w->field[0], nb * w->texel_bytes));
-    chk(hipMalloc(&w->field[1], nb * w->texel_bytes));
-    chk(hipMalloc(&w->empty_bits, ((nb + 31) / 32) * sizeof(uint32_t)));
-    chk(hipMalloc(&w->pal_bits, (size_t)slots * 128 * sizeof(uint32_t)));
-    chk(hipMalloc(&w->batch_counter, 2 * sizeof(unsigned int)));
     chk(hipHostMalloc((void**)&w->stage_blocks, nb * sizeof(int16_t), hipHostMallocDefault));
     chk(hipEventCreateWithFlags(&w->stage_done, hipEventDisableTiming));
     chk(hipHostMalloc((void**)&w->stage_copy_src, (size_t)slots * sizeof(int32_t), hipHostMallocDefault));
-    w->device_bytes = nb * sizeof(int16_t) + (size_t)slots * 4096 + 256 * 6 * sizeof(float) +
-                      (size_t)slots * sizeof(int32_t) + 2 * nb * w->texel_bytes + ((nb + 31) / 32) * 4 +
-                      (size_t)slots * 128 * 4 + 8;
-    if (e == hipSuccess) {
-        chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
-        chk(hipMemset(w->pal_bits, 0, (size_t)slots * 128 * sizeof(uint32_t)));
-        chk(hipMemset(w->batch_counter, 0, 2 * sizeof(unsigned int)));
-        chk(hipMemset(w->blocks, 0, nb * sizeof(int16_t)));
-        chk(hipMemset(w->palette, 0, (size_t)slots * 4096));
-        chk(hipMemset(w->mats, 0, 256 * 6 * sizeof(float)));
-        chk(hipMemset(w->copy_src_dev, 0, (size_t)slots * sizeof(int32_t)));
-        chk(hipMemset(w->field[0], 0, nb * w->texel_bytes));
-        chk(hipMemset(w->field[1], 0, nb * w->texel_bytes));
-        for (int p = 0; p < 3; ++p)
-            for (int k = 0; k < 2; ++k) chk(hipEventCreate(&w->ev[p][k]));
-    }
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < 2; ++k) chk(hipEventCreate(&w->ev[p][k]));
     if (e != hipSuccess) {
         world_free(w);
         delete w;
@@ -412,6 +441,7 @@ int lumx_world_get_block(lumx_world_t w, int x, int y, int z, int16_t* out) {
 
 int lumx_world_set_palette(lumx_world_t w, int first, int count, const uint8_t* vox, int layout) {
     if (!w || (!vox && count > 0)) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     if (first < 0 || count < 0 || first + count > w->desc.static_blocks)
         return fail(LUMX_EINVAL, "palette range [%d, %d) outside the static palette [0, %d)", first, first + count,
                     w->desc.static_blocks);
@@ -445,6 +475,7 @@ int lumx_world_set_palette(lumx_world_t w, int first, int count, const uint8_t* 
 
 int lumx_world_get_palette(lumx_world_t w, int first, int count, uint8_t* out) {
     if (!w || (!out && count > 0)) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     if (first < 0 || count < 0 || first + count > w->desc.palette_slots)
         return fail(LUMX_EINVAL, "palette range outside [0, %d)", w->desc.palette_slots);
     HIP_TRY(hipSetDevice(w->ctx->device));
@@ -456,6 +487,7 @@ int lumx_world_get_palette(lumx_world_t w, int first, int count, uint8_t* out) {
 
 int lumx_world_set_materials(lumx_world_t w, const float* m) {
     if (!w || !m) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     HIP_TRY(hipSetDevice(w->ctx->device));
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(w->mats, m, 256 * 6 * sizeof(float), hipMemcpyHostToDevice));
@@ -478,6 +510,7 @@ static float host_bf16_to_f32(uint16_t h) {
 
 int lumx_world_write_radiance(lumx_world_t w, const float* rgba) {
     if (!w || !rgba) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     size_t n = (size_t)w->n_blocks;
     HIP_TRY(hipSetDevice(w->ctx->device));
     HIP_TRY(hipDeviceSynchronize());
@@ -500,6 +533,7 @@ int lumx_world_write_radiance(lumx_world_t w, const float* rgba) {
 
 int lumx_world_read_radiance(lumx_world_t w, float* rgba) {
     if (!w || !rgba) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     size_t n = (size_t)w->n_blocks;
     HIP_TRY(hipSetDevice(w->ctx->device));
     HIP_TRY(hipDeviceSynchronize());
@@ -523,6 +557,7 @@ int lumx_world_read_radiance(lumx_world_t w, float* rgba) {
 
 int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes) {
     if (!w || !ptr || !bytes) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     *ptr = w->field[w->cur];
     *bytes = (size_t)w->n_blocks * w->texel_bytes;
     return LUMX_OK;
@@ -577,6 +612,7 @@ static int commit_palette_bits(lumx_world* w, hipStream_t s) {
 
 int lumx_world_upload_blocks(lumx_world_t w, void* stream) {
     if (!w) return fail(LUMX_EINVAL, "null world");
+    REQUIRE_RESIDENT(w);
     HIP_TRY(hipSetDevice(w->ctx->device));
     w->current = w->origin;
     w->counter = w->desc.static_blocks;
@@ -588,6 +624,60 @@ int lumx_world_upload_blocks(lumx_world_t w, void* stream) {
     if (stream_capturing(s)) return LUMX_OK;
     HIP_TRY(hipEventRecord(w->stage_done, s));
     w->stage_pending = true;
+    return LUMX_OK;
+}
+
+int lumx_world_evict(lumx_world_t w) {
+    if (!w) return fail(LUMX_EINVAL, "null world");
+    if (!w->blocks) return LUMX_OK;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
+    w->stage_pending = false;
+    const size_t nb = (size_t)w->n_blocks;
+    const size_t used = (size_t)std::max(w->counter, w->prev_counter);
+    w->snap_field.resize(nb * w->texel_bytes);
+    w->snap_palette.resize(used * 4096);
+    w->snap_mats.resize(256 * 6);
+    // Blocking copies on the null stream: the world's own work has completed
+    // (the caller evicts idle worlds only).
+    HIP_TRY(hipMemcpy(w->snap_field.data(), w->field[w->cur], nb * w->texel_bytes, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(w->snap_palette.data(), w->palette, used * 4096, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(w->snap_mats.data(), w->mats, 256 * 6 * sizeof(float), hipMemcpyDeviceToHost));
+    world_free_device(w);
+    return LUMX_OK;
+}
+
+int lumx_world_restore(lumx_world_t w) {
+    if (!w) return fail(LUMX_EINVAL, "null world");
+    if (w->blocks) return LUMX_OK;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipError_t e = world_alloc_device(w);
+    if (e != hipSuccess) {
+        world_free_device(w);
+        return fail(e == hipErrorOutOfMemory ? LUMX_ENOMEM : LUMX_EDEVICE, "restore: %s", hipGetErrorString(e));
+    }
+    const size_t nb = (size_t)w->n_blocks;
+    const int used = (int)(w->snap_palette.size() / 4096);
+    HIP_TRY(hipMemcpy(w->blocks, w->current.data(), nb * 2, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(w->palette, w->snap_palette.data(), w->snap_palette.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(w->mats, w->snap_mats.data(), 256 * 6 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(w->field[w->cur], w->snap_field.data(), nb * w->texel_bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(w->copy_src_dev, w->copy_src.data(), (size_t)w->desc.palette_slots * 4, hipMemcpyHostToDevice));
+    HIP_TRY(launch_block_bits(w->blocks, w->n_blocks, w->empty_bits, nullptr));
+    HIP_TRY(launch_palette_bits(w->palette, 0, used, w->pal_bits, nullptr));
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    w->snap_field.clear();
+    w->snap_field.shrink_to_fit();
+    w->snap_palette.clear();
+    w->snap_palette.shrink_to_fit();
+    w->snap_mats.clear();
+    return LUMX_OK;
+}
+
+int lumx_world_resident(lumx_world_t w, int* resident, size_t* device_bytes) {
+    if (!w) return fail(LUMX_EINVAL, "null world");
+    if (resident) *resident = w->blocks != nullptr;
+    if (device_bytes) *device_bytes = w->device_bytes;
     return LUMX_OK;
 }
 
@@ -712,6 +802,7 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, co
     if (!w || n < 0 || (n > 0 && (!models || !pl))) return fail(LUMX_EINVAL, "bad argument");
     for (int i = 0; i < n; ++i)
         if (!models[i] || models[i]->ctx != w->ctx) return fail(LUMX_EINVAL, "model %d invalid", i);
+    REQUIRE_RESIDENT(w);
     HIP_TRY(hipSetDevice(w->ctx->device));
     hipStream_t s = pick_stream(w->ctx, stream);
     // start_blockify: current = origin, counter = static.
@@ -772,6 +863,7 @@ int lumx_map_models(lumx_world_t w, int n, const lumx_model_t* models, const lum
 extern "C++" {
 int lumx::radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p) {
     if (!w || !p) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     if (!valid_rays(p->rays))
         return fail(LUMX_EINVAL, "rays=%d: must be a power of two <= 64 or a multiple of 64 <= 1024", p->rays);
     if (p->size < 1 || p->shift < 0 || p->shift >= p->size)
@@ -1045,6 +1137,7 @@ void lumx::glossy_args(lumx_world* w, const uint8_t* matnorm, const float* depth
 int lumx_glossy(lumx_world_t w, const uint8_t* matnorm, const float* depth, int width, int height,
                 const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys, void* stream) {
     if (!w || !matnorm || !depth || !cam || !out) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
     if (width <= 0 || height <= 0 || width > 16384 || height > 16384)
         return fail(LUMX_EINVAL, "frame %dx%d invalid", width, height);
     if (!(max_dist > 0.0f && max_dist <= 65536.0f) || max_steps < 0)

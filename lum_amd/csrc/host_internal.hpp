@@ -69,6 +69,9 @@ struct lumx_world {
     hipEvent_t stage_done = nullptr;    // recorded after the staging uploads of the last map
     bool stage_pending = false;
     std::vector<void*> graph_stage;     // pinned staging owned by maps recorded into graphs
+    // Host snapshot of the device state while evicted (blocks live in `current`).
+    std::vector<uint8_t> snap_field, snap_palette;
+    std::vector<float> snap_mats;
     int32_t* stage_copy_src = nullptr;  // pinned host staging of the copy list
     lumx_radiance_tuning tune{};
     size_t device_bytes = 0;            // HBM held by this world
@@ -85,5 +88,17 @@ void glossy_args(lumx_world* w, const uint8_t* matnorm, const float* depth, int 
                  const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys, GlossyArgs* a);
 WorldView view_of(const lumx_world* w);
 bool stream_capturing(hipStream_t s);
+
+// Batched launches over any set of distinct worlds (batch.cpp).
+struct Launcher;
+int launcher_create(lumx_context* ctx, int max_worlds, int ring_slots, Launcher** out);
+void launcher_destroy(Launcher* L);
+int launcher_set_frame(Launcher* L, int value, int add, hipStream_t s);
+int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32_t* const* requests,
+                      const int* n_requests, const lumx_radiance_params* params, bool use_frame, hipStream_t s,
+                      bool served);
+int launcher_glossy(Launcher* L, lumx_world* const* worlds, int n, const uint8_t* const* matnorm,
+                    const float* const* depth, const int* width, const int* height, const lumx_camera* cams,
+                    const float* max_dist, const int* max_steps, float* const* out, hipStream_t s);
 
 }  // namespace lumx

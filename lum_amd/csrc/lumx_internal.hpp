@@ -90,7 +90,7 @@ hipError_t launch_map(const MapArgs& a, hipStream_t stream);
 hipError_t launch_exec_copies(const CopyArgs& a, hipStream_t stream);
 hipError_t launch_glossy(const GlossyArgs& a, int store, bool traced, hipStream_t stream);
 hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
-                                 int total_threads, int block, int store, bool occ, const int* frame_dev,
+                                 int total_threads, int block, int store, bool served, const int* frame_dev,
                                  hipStream_t stream);
 hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_threads, int store,
                                hipStream_t stream);

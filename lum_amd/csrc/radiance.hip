@@ -154,7 +154,9 @@ __global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
 // (request index within the world's own update) and reduction order are those
 // of a single-world launch, so every world's result is the same alone and in
 // the batch.  time_seed of every world is offset by *frame (0 when null).
-template <int STORE, bool OCC>
+// SERVED only separates the serving engine's launches from the batch API's
+// in profiles; the code is the same.
+template <int STORE, bool OCC, bool SERVED>
 __global__ void __launch_bounds__(1024)
 radiance_batch_kernel(const RadianceArgs* args, const int* begin, int n_worlds, const int* frame) {
     extern __shared__ f3 chunk_sum[];
@@ -172,18 +174,18 @@ radiance_batch_kernel(const RadianceArgs* args, const int* begin, int n_worlds, 
 }
 
 hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
-                                 int total_threads, int block, int store, bool occ, const int* frame_dev,
+                                 int total_threads, int block, int store, bool served, const int* frame_dev,
                                  hipStream_t stream) {
     if (total_threads <= 0 || n_worlds <= 0) return hipSuccess;
     const int grid = (total_threads + block - 1) / block;
     const size_t lds = sizeof(f3) * (block / 64);
-#define LUMX_BATCH_LAUNCH(S, O) \
-    hipLaunchKernelGGL((radiance_batch_kernel<S, O>), dim3(grid), dim3(block), lds, stream, args_dev, begin_dev, \
-                       n_worlds, frame_dev)
+#define LUMX_BATCH_LAUNCH(S, V) \
+    hipLaunchKernelGGL((radiance_batch_kernel<S, false, V>), dim3(grid), dim3(block), lds, stream, args_dev, \
+                       begin_dev, n_worlds, frame_dev)
     switch (store) {
-        case STORE_F16: if (occ) LUMX_BATCH_LAUNCH(STORE_F16, true); else LUMX_BATCH_LAUNCH(STORE_F16, false); break;
-        case STORE_BF16: if (occ) LUMX_BATCH_LAUNCH(STORE_BF16, true); else LUMX_BATCH_LAUNCH(STORE_BF16, false); break;
-        default: if (occ) LUMX_BATCH_LAUNCH(STORE_F32, true); else LUMX_BATCH_LAUNCH(STORE_F32, false); break;
+        case STORE_F16: if (served) LUMX_BATCH_LAUNCH(STORE_F16, true); else LUMX_BATCH_LAUNCH(STORE_F16, false); break;
+        case STORE_BF16: if (served) LUMX_BATCH_LAUNCH(STORE_BF16, true); else LUMX_BATCH_LAUNCH(STORE_BF16, false); break;
+        default: if (served) LUMX_BATCH_LAUNCH(STORE_F32, true); else LUMX_BATCH_LAUNCH(STORE_F32, false); break;
     }
 #undef LUMX_BATCH_LAUNCH
     return hipGetLastError();

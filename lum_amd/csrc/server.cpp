@@ -1,0 +1,422 @@
+// server.cpp - continuous-batching request engine over resident worlds.
+//
+// One worker thread owns the device work of a server:
+//   1. it retires launches whose completion event has fired (latency is
+//      stamped there) and frees their worlds;
+//   2. while fewer than `streams` launches are in flight it builds the next
+//      launch from the pending queue in arrival order: a request is taken if
+//      its world is idle, not yet in this launch, no earlier pending request
+//      of the same world was passed over, and the world is (or can be made)
+//      resident within the HBM budget by evicting least-recently-used idle
+//      worlds;
+//   3. the taken radiance requests go into one batched radiance launch and
+//      the glossy requests into one batched glossy launch, on the next stream,
+//      followed by a completion event.
+// Requests are therefore overlapped across streams and packed into launches
+// as they arrive, while every request computes what it would alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "host_internal.hpp"
+
+using namespace lumx;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double seconds_since(Clock::time_point t0, Clock::time_point t) {
+    return std::chrono::duration<double>(t - t0).count();
+}
+
+struct Req {
+    lumx_request spec;
+    Clock::time_point t_submit, t_done;
+    bool done = false;
+    int status = LUMX_OK;
+};
+
+struct ServedWorld {
+    lumx_world* w = nullptr;
+    int busy = 0;
+    unsigned long long last_use = 0;
+};
+
+struct Flight {
+    std::vector<Req*> reqs;
+    std::vector<int> worlds;
+    hipEvent_t done = nullptr;
+    int stream = 0;
+};
+
+}  // namespace
+
+struct lumx_server {
+    lumx_context* ctx = nullptr;
+    lumx_server_desc desc{};
+    Launcher* L = nullptr;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> free_events;
+    int next_stream = 0;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::vector<ServedWorld> worlds;
+    std::deque<Req*> pending;
+    std::vector<std::unique_ptr<Req>> tickets;
+    std::vector<Flight> flights;
+    bool stop = false;
+    unsigned long long use_clock = 0;
+    size_t resident_bytes = 0;
+    lumx_server_stats stats{};
+    std::thread worker;
+};
+
+static bool resident(const lumx_world* w) { return w->blocks != nullptr; }
+
+// Make world i resident within the budget (caller holds the lock; `taken`
+// marks worlds already in the launch being built).  Returns false if the
+// budget cannot be met now.
+static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& taken) {
+    ServedWorld& sw = s->worlds[i];
+    if (resident(sw.w)) return true;
+    const size_t need = sw.w->device_bytes;
+    const size_t budget = s->desc.hbm_budget;
+    while (budget && s->resident_bytes + need > budget) {
+        int victim = -1;
+        for (int k = 0; k < (int)s->worlds.size(); ++k) {
+            const ServedWorld& o = s->worlds[k];
+            if (k == i || o.busy || taken[k] || !resident(o.w)) continue;
+            if (victim < 0 || o.last_use < s->worlds[victim].last_use) victim = k;
+        }
+        if (victim < 0) return false;
+        if (lumx_world_evict(s->worlds[victim].w) != LUMX_OK) return false;
+        s->resident_bytes -= s->worlds[victim].w->device_bytes;
+        s->stats.evictions++;
+    }
+    if (lumx_world_restore(sw.w) != LUMX_OK) return false;
+    s->resident_bytes += need;
+    s->stats.restores++;
+    return true;
+}
+
+static int launch_flight(lumx_server* s, Flight& f) {
+    hipStream_t st = s->streams[f.stream];
+    std::vector<lumx_world*> rw, gw;
+    std::vector<const int32_t*> rq;
+    std::vector<int> rn;
+    std::vector<lumx_radiance_params> rp;
+    std::vector<const uint8_t*> gm;
+    std::vector<const float*> gd;
+    std::vector<int> gwid, ght, gms;
+    std::vector<float> gmd;
+    std::vector<lumx_camera> gc;
+    std::vector<float*> go;
+    for (size_t k = 0; k < f.reqs.size(); ++k) {
+        const lumx_request& r = f.reqs[k]->spec;
+        lumx_world* w = s->worlds[f.worlds[k]].w;
+        if (r.kind == LUMX_REQ_RADIANCE) {
+            rw.push_back(w);
+            rq.push_back(r.requests_dev);
+            rn.push_back(r.requests_dev ? r.n_requests : 0);
+            rp.push_back(r.params);
+        } else {
+            gw.push_back(w);
+            gm.push_back(r.matnorm_dev);
+            gd.push_back(r.depth_dev);
+            gwid.push_back(r.width);
+            ght.push_back(r.height);
+            gc.push_back(r.cam);
+            gmd.push_back(r.max_dist);
+            gms.push_back(r.max_steps);
+            go.push_back(r.out_dev);
+        }
+    }
+    bool any_list = false;
+    for (const int32_t* p : rq) any_list |= (p != nullptr);
+    int rc = launcher_radiance(s->L, rw.data(), (int)rw.size(), any_list ? rq.data() : nullptr, rn.data(), rp.data(),
+                               false, st, true);
+    if (rc) return rc;
+    rc = launcher_glossy(s->L, gw.data(), (int)gw.size(), gm.data(), gd.data(), gwid.data(), ght.data(), gc.data(),
+                         gmd.data(), gms.data(), go.data(), st);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(f.done, st));
+    return LUMX_OK;
+}
+
+static void worker_loop(lumx_server* s) {
+    hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        // 1. retire finished launches
+        bool retired = false;
+        for (size_t k = 0; k < s->flights.size();) {
+            Flight& f = s->flights[k];
+            if (hipEventQuery(f.done) == hipSuccess) {
+                const Clock::time_point now = Clock::now();
+                for (size_t j = 0; j < f.reqs.size(); ++j) {
+                    Req* r = f.reqs[j];
+                    r->t_done = now;
+                    r->done = true;
+                    s->worlds[f.worlds[j]].busy--;
+                    s->stats.completed++;
+                    if (r->spec.kind == LUMX_REQ_RADIANCE) {
+                        const lumx_world* w = s->worlds[f.worlds[j]].w;
+                        long long n = r->spec.requests_dev ? r->spec.n_requests : w->n_blocks;
+                        s->stats.probe_rays += (n / r->spec.params.size) * (long long)r->spec.params.rays;
+                    } else {
+                        s->stats.pixels += (long long)r->spec.width * r->spec.height;
+                    }
+                }
+                s->free_events.push_back(f.done);
+                s->flights.erase(s->flights.begin() + k);
+                retired = true;
+            } else {
+                ++k;
+            }
+        }
+        if (retired) s->cv_done.notify_all();
+        if (s->stop && s->pending.empty() && s->flights.empty()) break;
+
+        // 2. build the next launch
+        bool launched = false;
+        if ((int)s->flights.size() < s->desc.streams && !s->pending.empty()) {
+            const int nw = (int)s->worlds.size();
+            std::vector<char> taken(nw, 0), blocked(nw, 0);
+            Flight f;
+            for (auto it = s->pending.begin(); it != s->pending.end() && (int)f.reqs.size() < s->desc.max_batch;) {
+                Req* r = *it;
+                const int wi = r->spec.world;
+                bool ok = !blocked[wi] && !taken[wi] && s->worlds[wi].busy == 0;
+                if (ok && !f.reqs.empty()) {
+                    // one radiance format per launch
+                    ok = s->worlds[wi].w->desc.radiance_format == s->worlds[f.worlds[0]].w->desc.radiance_format;
+                }
+                if (ok) ok = ensure_resident(s, wi, taken);
+                if (!ok) {
+                    blocked[wi] = 1;
+                    ++it;
+                    continue;
+                }
+                taken[wi] = 1;
+                s->worlds[wi].busy++;
+                s->worlds[wi].last_use = ++s->use_clock;
+                f.reqs.push_back(r);
+                f.worlds.push_back(wi);
+                it = s->pending.erase(it);
+            }
+            if (!f.reqs.empty()) {
+                if (s->free_events.empty()) {
+                    hipEvent_t e;
+                    hipEventCreateWithFlags(&e, hipEventDisableTiming);
+                    s->free_events.push_back(e);
+                }
+                f.done = s->free_events.back();
+                s->free_events.pop_back();
+                f.stream = s->next_stream;
+                s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
+                int rc = launch_flight(s, f);
+                s->stats.launches++;
+                if (rc != LUMX_OK) {
+                    // Record the failure on the requests; their worlds are released.
+                    const Clock::time_point now = Clock::now();
+                    for (size_t j = 0; j < f.reqs.size(); ++j) {
+                        f.reqs[j]->status = rc;
+                        f.reqs[j]->t_done = now;
+                        f.reqs[j]->done = true;
+                        s->worlds[f.worlds[j]].busy--;
+                    }
+                    s->free_events.push_back(f.done);
+                    s->cv_done.notify_all();
+                } else {
+                    s->flights.push_back(std::move(f));
+                }
+                launched = true;
+            }
+        }
+        if (launched) continue;
+        // 3. wait: poll in-flight events briefly, or sleep until a submission
+        if (!s->flights.empty()) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(5));
+            lk.lock();
+        } else {
+            s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
+        }
+    }
+}
+
+extern "C" {
+
+int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_server_t* out) {
+    if (!ctx || !d || !out) return fail(LUMX_EINVAL, "null argument");
+    if (d->max_batch < 1 || d->max_batch > 4096 || d->streams < 1 || d->streams > 16)
+        return fail(LUMX_EINVAL, "max_batch must be in [1, 4096] and streams in [1, 16]");
+    HIP_TRY(hipSetDevice(ctx->device));
+    lumx_server* s = new lumx_server();
+    s->ctx = ctx;
+    s->desc = *d;
+    int rc = launcher_create(ctx, d->max_batch, 4 * d->streams + 4, &s->L);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    for (int k = 0; k < d->streams; ++k) {
+        hipStream_t st;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            for (hipStream_t t : s->streams) hipStreamDestroy(t);
+            launcher_destroy(s->L);
+            delete s;
+            return fail(LUMX_EDEVICE, "stream creation failed");
+        }
+        s->streams.push_back(st);
+    }
+    s->worker = std::thread(worker_loop, s);
+    *out = s;
+    return LUMX_OK;
+}
+
+int lumx_server_destroy(lumx_server_t s) {
+    if (!s) return fail(LUMX_EINVAL, "null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->worker.join();
+    hipSetDevice(s->ctx->device);
+    for (hipStream_t t : s->streams) {
+        hipStreamSynchronize(t);
+        hipStreamDestroy(t);
+    }
+    for (hipEvent_t e : s->free_events) hipEventDestroy(e);
+    launcher_destroy(s->L);
+    delete s;
+    return LUMX_OK;
+}
+
+int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
+    if (!s || !w || !id) return fail(LUMX_EINVAL, "null argument");
+    if (w->ctx != s->ctx) return fail(LUMX_EINVAL, "world belongs to another context");
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const ServedWorld& o : s->worlds)
+        if (o.w == w) return fail(LUMX_EINVAL, "world already added");
+    ServedWorld sw;
+    sw.w = w;
+    s->worlds.push_back(sw);
+    if (resident(w)) {
+        s->resident_bytes += w->device_bytes;
+        if (s->desc.hbm_budget && s->resident_bytes > s->desc.hbm_budget) {
+            // Over budget: this world waits on the host until a request needs it.
+            if (lumx_world_evict(w) == LUMX_OK) {
+                s->resident_bytes -= w->device_bytes;
+                s->stats.evictions++;
+            }
+        }
+    }
+    *id = (int)s->worlds.size() - 1;
+    return LUMX_OK;
+}
+
+int lumx_server_submit(lumx_server_t s, const lumx_request* r, long long* ticket) {
+    if (!s || !r || !ticket) return fail(LUMX_EINVAL, "null argument");
+    if (r->kind != LUMX_REQ_RADIANCE && r->kind != LUMX_REQ_GLOSSY) return fail(LUMX_EINVAL, "request kind invalid");
+    std::unique_ptr<Req> q(new Req());
+    q->spec = *r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->world < 0 || r->world >= (int)s->worlds.size()) return fail(LUMX_EINVAL, "world id %d invalid", r->world);
+        if (r->kind == LUMX_REQ_RADIANCE) {
+            const int rays = r->params.rays;
+            if (rays <= 0 || rays > 1024 || (rays & (rays - 1)))
+                return fail(LUMX_EINVAL, "served radiance rays must be a power of two <= 1024");
+            if (r->requests_dev && r->n_requests < 0) return fail(LUMX_EINVAL, "n_requests invalid");
+        } else if (!r->matnorm_dev || !r->depth_dev || !r->out_dev || r->width <= 0 || r->height <= 0) {
+            return fail(LUMX_EINVAL, "glossy request buffers or size invalid");
+        }
+        q->t_submit = Clock::now();
+        *ticket = (long long)s->tickets.size();
+        s->pending.push_back(q.get());
+        s->tickets.push_back(std::move(q));
+        s->stats.submitted++;
+    }
+    s->cv_work.notify_one();
+    return LUMX_OK;
+}
+
+int lumx_server_wait(lumx_server_t s, long long ticket, double* latency_s) {
+    if (!s) return fail(LUMX_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (ticket < 0 || ticket >= (long long)s->tickets.size()) return fail(LUMX_EINVAL, "ticket invalid");
+    Req* r = s->tickets[ticket].get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (latency_s) *latency_s = seconds_since(r->t_submit, r->t_done);
+    if (r->status != LUMX_OK) return fail(r->status, "request %lld failed", ticket);
+    return LUMX_OK;
+}
+
+int lumx_server_drain(lumx_server_t s) {
+    if (!s) return fail(LUMX_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->flights.empty(); });
+    return LUMX_OK;
+}
+
+int lumx_server_get_stats(lumx_server_t s, lumx_server_stats* out) {
+    if (!s || !out) return fail(LUMX_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->resident_bytes = s->resident_bytes;
+    return LUMX_OK;
+}
+
+int lumx_server_run_trace(lumx_server_t s, int n, const lumx_request* reqs, const double* arrival_s,
+                          double* latency_s, double* wall_s) {
+    if (!s || n < 0 || (n > 0 && (!reqs || !arrival_s))) return fail(LUMX_EINVAL, "bad argument");
+    for (int i = 1; i < n; ++i)
+        if (arrival_s[i] < arrival_s[i - 1]) return fail(LUMX_EINVAL, "arrival times must be non-decreasing");
+    std::vector<long long> tickets(n, -1);
+    std::atomic<int> submit_rc(LUMX_OK);
+    const Clock::time_point t0 = Clock::now();
+    std::thread submitter([&] {
+        for (int i = 0; i < n; ++i) {
+            std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
+                                                    std::chrono::duration<double>(arrival_s[i])));
+            int rc = lumx_server_submit(s, &reqs[i], &tickets[i]);
+            if (rc != LUMX_OK) {
+                submit_rc = rc;
+                return;
+            }
+        }
+    });
+    submitter.join();
+    if (submit_rc != LUMX_OK) {
+        lumx_server_drain(s);
+        return submit_rc;
+    }
+    int status = LUMX_OK;
+    Clock::time_point last = t0;
+    for (int i = 0; i < n; ++i) {
+        double lat = 0.0;
+        int rc = lumx_server_wait(s, tickets[i], &lat);
+        if (rc != LUMX_OK) status = rc;
+        if (latency_s) latency_s[i] = lat;
+        std::lock_guard<std::mutex> lk(s->mu);
+        last = std::max(last, s->tickets[tickets[i]]->t_done);
+    }
+    if (wall_s) *wall_s = seconds_since(t0 + std::chrono::duration_cast<Clock::duration>(
+                                                  std::chrono::duration<double>(n ? arrival_s[0] : 0.0)),
+                                        last);
+    return status;
+}
+
+}  // extern "C"
