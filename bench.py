@@ -454,12 +454,26 @@ def run_serving(args):
                      "bytes_per_launch": bytes_per_launch,
                      "launch": "all %d worlds, every probe, 64 rays, FP16" % n_worlds},
     }
+    if args.copy_calibration_mb > 0:
+        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration_mb)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_serving(trace, hosts, perm_host, sizes, seconds=args.cpu_seconds)
     for w in worlds:
         w.close()
     ctx.close()
     return out
+
+
+def copy_calibration(mb):
+    """A plain device copy of known size, for scaling PMC byte counters."""
+    import torch
+    n = int(mb * (1 << 20) // 4)
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return n * 4
 
 
 def cpu_baseline_serving(trace, hosts, perm_host, sizes, seconds=10.0):

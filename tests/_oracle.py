@@ -8,7 +8,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liblum_oracle.so")
+ORACLE_SO = os.environ.get("LUMX_ORACLE_SO") or os.path.join(ORACLE_DIR, "liblum_oracle.so")
 
 
 class OrcWorld(C.Structure):

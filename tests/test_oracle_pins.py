@@ -1,0 +1,528 @@
+"""Pins of the CPU oracle (no GPU): closed forms, invariants and brute-force
+cases that a plausible restatement mistake (dropped term, wrong sign or
+index, transposed operand, wrong truncation) would fail.
+
+Reference semantics pinned here (paths relative to the reference repository):
+  shaders/radiance.comp:51-66 hash33, :293-306 sphericalFibonacci,
+  :97-143 GetVoxel, :161-166 sample_radiance, :183-291 CastRay_fast,
+  :316-354 trace_ray + store_probe; shaders/map.comp:33-68;
+  shaders/glossy.frag:312-360,444-493;
+  src/renderer/src/internal_render.cpp:1029-1076 (blockify), 1142-1168 (probe
+  selection), 1485-1518 (map_mesh placement), 923-934 (camera).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import tests._oracle as orc
+
+SKY = np.float32([0.53, 0.81, 0.92]) * np.float32(0.1)
+
+
+def world(blocks, palette, mats=None):
+    if mats is None:
+        mats = np.zeros((256, 6), np.float32)
+    return orc.WorldData(blocks, palette, mats)
+
+
+# ---------------------------------------------------------------- hash33
+def test_hash33_zero_is_fixed_point():
+    assert np.array_equal(orc.hash33_u([0, 0, 0]), [0, 0, 0])
+    assert np.array_equal(orc.hash33([0, 0, 0]), [0, 0, 0])
+
+
+def test_hash33_matches_vectorised_glsl_semantics():
+    """x = ((x >> 8) ^ x.yzx) * 1103515245u, three rounds, on uvec3."""
+    rng = np.random.default_rng(0)
+    seeds = rng.integers(0, 2 ** 32, size=(2000, 3), dtype=np.uint64).astype(np.uint32)
+    x = seeds.copy()
+    k = np.uint32(1103515245)
+    with np.errstate(over="ignore"):
+        for _ in range(3):
+            yzx = np.roll(x, -1, axis=1)  # .yzx swizzle
+            x = ((x >> np.uint32(8)) ^ yzx) * k
+    for i in range(0, 2000, 97):
+        assert np.array_equal(orc.hash33_u(seeds[i]), x[i])
+        f = orc.hash33(seeds[i])
+        # vec3(x) * (1.0 / float(0xffffffffu)): float(0xffffffff) rounds to 2^32
+        assert np.array_equal(f, x[i].astype(np.float32) * np.float32(2.0 ** -32))
+
+
+def test_hash33_range_and_mean():
+    vals = np.array([orc.hash33([i, 7, 7]) for i in range(4000)])
+    assert vals.min() >= 0.0 and vals.max() <= 1.0
+    assert abs(vals.mean() - 0.5) < 0.02
+
+
+# ---------------------------------------------------------------- fibonacci
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_fibonacci_closed_form(n):
+    phi_g = (math.sqrt(5.0) * 0.5 + 0.5) - 1.0
+    dirs = np.array([orc.fibonacci(i, n) for i in range(n)], np.float64)
+    for i, d in enumerate(dirs):
+        # z = cos(theta) = 1 - (2i + 1)/n, evaluated in FP32
+        ct = np.float32(1.0) - (np.float32(2.0) * np.float32(i) + np.float32(1.0)) * (np.float32(1.0) / np.float32(n))
+        assert d[2] == ct
+        assert abs(np.linalg.norm(d) - 1.0) < 2e-6
+        a = (i * phi_g) % 1.0 * 2 * math.pi
+        az = math.atan2(d[1], d[0]) % (2 * math.pi)
+        diff = min(abs(az - a), 2 * math.pi - abs(az - a))
+        if np.hypot(d[0], d[1]) > 1e-3:
+            assert diff < 1e-4, (i, az, a)
+    # a spherical Fibonacci set is balanced
+    assert np.linalg.norm(dirs.sum(0)) < 0.05 * n ** 0.5
+
+
+# ---------------------------------------------------------------- trilinear
+def field_of(wx, wy, wz, rng=None, value=None):
+    f = np.zeros((wz, wy, wx, 4), np.float32)
+    if value is not None:
+        f[..., :3] = value
+    elif rng is not None:
+        f[..., :3] = rng.random((wz, wy, wx, 3))
+    f[..., 3] = 1
+    return f
+
+
+def test_sample_radiance_texel_centres_and_repeat():
+    rng = np.random.default_rng(1)
+    f = field_of(5, 4, 3, rng)
+    n0 = [0.0, 0.0, 0.0]
+    # x at texel centre (bp = x/16 + n): bp integer -> exact texel
+    for (tx, ty, tz) in [(0, 0, 0), (4, 3, 2), (2, 1, 1)]:
+        got = orc.sample_radiance(f, [16.0 * tx, 16.0 * ty, 16.0 * tz], n0)
+        assert np.array_equal(got, f[tz, ty, tx, :3])
+    # REPEAT: bp.x = -1 reads texel x = W-1; bp = W reads texel 0
+    got = orc.sample_radiance(f, [-16.0, 16.0, 0.0], n0)
+    assert np.array_equal(got, f[0, 1, 4, :3])
+    got = orc.sample_radiance(f, [16.0 * 5, 0.0, 16.0 * 3], n0)
+    assert np.array_equal(got, f[0, 0, 0, :3])
+    # normal offsets by one texel: x + n*16
+    got = orc.sample_radiance(f, [16.0, 0.0, 0.0], [1.0, 0.0, 0.0])
+    assert np.array_equal(got, f[0, 0, 2, :3])
+    # midpoint between x-neighbours is their mean
+    got = orc.sample_radiance(f, [16.0 * 1.5, 16.0, 16.0], n0)
+    assert np.allclose(got, 0.5 * (f[1, 1, 1, :3] + f[1, 1, 2, :3]), atol=1e-6)
+
+
+def test_sample_radiance_is_linear_and_normalised():
+    rng = np.random.default_rng(2)
+    f1, f2 = field_of(6, 5, 4, rng), field_of(6, 5, 4, rng)
+    a, b = 0.25, 2.0
+    fab = f1.copy()
+    fab[..., :3] = a * f1[..., :3] + b * f2[..., :3]
+    for _ in range(20):
+        x = rng.uniform(-40, 120, 3)
+        n = rng.uniform(-1, 1, 3)
+        s = a * orc.sample_radiance(f1, x, n) + b * orc.sample_radiance(f2, x, n)
+        assert np.allclose(orc.sample_radiance(fab, x, n), s, rtol=1e-5, atol=1e-6)
+    u = field_of(6, 5, 4, value=0.3)
+    for _ in range(10):
+        assert np.allclose(orc.sample_radiance(u, rng.uniform(-50, 150, 3), rng.uniform(-1, 1, 3)), 0.3, atol=2e-7)
+
+
+# ---------------------------------------------------------------- march
+def solid_world(wx, wy, wz, voxel=5):
+    blocks = np.ones((wz, wy, wx), np.int16)
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    pal[1] = voxel
+    return blocks, pal
+
+
+def test_ray_in_solid_world_hits_at_first_sample_with_zero_fraction():
+    blocks, pal = solid_world(3, 3, 3)
+    w = world(blocks, pal)
+    f = field_of(3, 3, 3, value=0.5)
+    r = orc.trace_ray(w, f, [24.3, 24.7, 24.1], [0.48, -0.6, 0.64])
+    # first coarse sample is solid; the refinement's start voxel
+    # (origin + d*(0.5 - 1.5)) is solid too: normal 0, fraction 0
+    assert r["hit"] and r["coarse"] == 1 and r["refine"] == 1
+    assert r["fraction"] == 0.0 and np.all(r["normal"] == 0)
+    assert r["voxel"] == 5
+
+
+def single_voxel_world(v, wx=4, wy=4, wz=4, material=9):
+    """All blocks non-empty but air except voxel v (no block skipping)."""
+    blocks = np.ones((wz, wy, wx), np.int16)
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    blocks[v[2] // 16, v[1] // 16, v[0] // 16] = 1
+    pal[1][v[2] % 16, v[1] % 16, v[0] % 16] = material
+    # every other block id must not be 0 (no skipping) and must be empty:
+    blocks[blocks == 1] = 2
+    blocks[v[2] // 16, v[1] // 16, v[0] // 16] = 1
+    pal = np.concatenate([pal, np.zeros((1, 16, 16, 16), np.uint8)])
+    return blocks, pal
+
+
+def key_of(v):
+    return 1 + ((v[0] + 512) & 1023) + (((v[1] + 512) & 1023) << 10) + (((v[2] + 512) & 1023) << 20)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_ray_aimed_at_single_voxel_hits_it_with_entry_face_normal(seed):
+    rng = np.random.default_rng(seed)
+    v = rng.integers(10, 54, 3)
+    blocks, pal = single_voxel_world(v)
+    w = world(blocks, pal)
+    f = field_of(4, 4, 4, value=0.25)
+    target = v + 0.5
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    o = target - d * rng.uniform(4.0, 8.0)
+    r = orc.trace_ray(w, f, o.astype(np.float32), d.astype(np.float32))
+    assert r["hit"]
+    assert r["key"] == key_of(v)
+    # exact entry face: the axis whose slab is entered last
+    t_enter = [((v[a] + (0 if d[a] > 0 else 1)) - o[a]) / d[a] for a in range(3)]
+    ax = int(np.argmax(t_enter))
+    expect = np.zeros(3)
+    expect[ax] = -np.sign(d[ax])
+    if r["refine"] > 1:  # the refinement stepped into the voxel through a face
+        assert np.array_equal(r["normal"], expect.astype(np.float32))
+
+
+def test_dda_ties_step_both_axes():
+    """CastRay_fast's step mask uses <= on every comparison, so when tMax.x
+    equals tMax.y both axes step at once: a ray in the plane x == y (equal
+    origin coordinates, equal direction components) stays on the diagonal
+    voxels and enters through a two-axis normal or along z."""
+    for oz in (3.25, 5.5, 7.75, 9.125):
+        v = (20, 20, 20)
+        blocks, pal = single_voxel_world(np.array(v))
+        w = world(blocks, pal)
+        d = np.float32([1.0, 1.0, 0.3])
+        d = d / np.float32(np.sqrt(np.float32(d @ d)))
+        o = np.float32([15.3, 15.3, 0.0])
+        t = (np.float32(20.5) - o[0]) / d[0]
+        o[2] = np.float32(20.5) - d[2] * t + np.float32(oz - 6.0) * np.float32(0.05)
+        r = orc.trace_ray(w, field_of(4, 4, 4, value=0.0), o, d)
+        assert r["hit"] and r["key"] == key_of(v)
+        assert r["normal"][0] == r["normal"][1]
+
+
+def test_empty_block_skipping_never_passes_a_solid_voxel():
+    """Air blocks (id 0) between origin and one solid voxel: the skip lands
+    the march in the voxel's block and it is found."""
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        blocks = np.zeros((6, 6, 6), np.int16)
+        v = rng.integers(20, 76, 3)
+        pal = np.zeros((2, 16, 16, 16), np.uint8)
+        pal[1] = 7  # the whole block is solid
+        blocks[v[2] // 16, v[1] // 16, v[0] // 16] = 1
+        w = world(blocks, pal)
+        o = rng.uniform(2, 94, 3)
+        d = (v + 0.5) - o
+        if np.linalg.norm(d) < 20:
+            continue
+        d /= np.linalg.norm(d)
+        r = orc.trace_ray(w, field_of(6, 6, 6, value=0.1), o.astype(np.float32), d.astype(np.float32))
+        assert r["hit"] and r["voxel"] == 7
+
+
+def test_empty_world_misses_with_sky_light():
+    blocks = np.zeros((4, 4, 4), np.int16)
+    pal = np.zeros((1, 16, 16, 16), np.uint8)
+    w = world(blocks, pal)
+    d = np.float32([0.0, 0.6, 0.8])  # far from the sun direction
+    r = orc.trace_ray(w, field_of(4, 4, 4, value=0.7), [30.0, 30.0, 30.0], d)
+    assert not r["hit"]
+    assert np.allclose(r["light"], SKY, atol=1e-7)
+    # looking into the sun (-globalLightDir) adds s * (0.9, 0.9, 0.6) * 3
+    L = np.array([0.5, 0.5, -0.9], np.float64)
+    L /= np.linalg.norm(L)
+    r = orc.trace_ray(w, field_of(4, 4, 4, value=0.7), [30.0, 30.0, 30.0], (-L).astype(np.float32))
+    assert np.allclose(r["light"], SKY + np.float32([0.9, 0.9, 0.6]) * 3.0, atol=1e-5)
+
+
+def test_max_steps_caps_the_march():
+    # non-empty but air blocks everywhere: samples every 0.5 voxel, no skip
+    blocks = np.ones((8, 8, 8), np.int16)
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    w = world(blocks, pal)
+    r = orc.trace_ray(w, field_of(8, 8, 8, value=0.0), [4.0, 4.0, 4.0], [1.0, 0.0, 0.0], max_steps=17)
+    assert not r["hit"] and r["coarse"] == 17
+    r = orc.trace_ray(w, field_of(8, 8, 8, value=0.0), [4.0, 4.0, 4.0], [1.0, 0.0, 0.0], max_dist=10.0)
+    assert not r["hit"] and r["coarse"] == 20  # fraction 0.5 .. 10.0
+
+
+def test_hit_test_precedes_bounds_test_with_truncation_quirk():
+    """ivec3(p) truncates toward zero, so a sample at p.x in (-1, 0) reads
+    voxel x = 0; the hit test runs before the bounds test, so the ray hits a
+    voxel although its sample lies outside the world."""
+    blocks = np.ones((1, 1, 1), np.int16)
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    pal[1][8, 8, 0] = 3  # voxel (0, 8, 8)
+    w = world(blocks, pal)
+    d = [-1.0, 0.001, 0.002]  # no exactly-zero component (those make the DDA's tMax inf/NaN)
+    r = orc.trace_ray(w, field_of(1, 1, 1, value=0.0), [0.3, 8.5, 8.5], d)
+    assert r["hit"] and r["coarse"] == 1 and r["voxel"] == 3
+    assert r["key"] == key_of((0, 8, 8)) and np.array_equal(r["normal"], [1.0, 0.0, 0.0])
+    # a sample outside the world that reads air ends the march as a miss
+    r = orc.trace_ray(w, field_of(1, 1, 1, value=0.0), [0.3, 4.5, 8.5], d)
+    assert not r["hit"] and r["coarse"] == 1
+
+
+# ---------------------------------------------------------------- radiance update
+def test_blend_is_linear_in_alpha_and_clamps_old():
+    rng = np.random.default_rng(3)
+    blocks = np.zeros((4, 4, 4), np.int16)
+    blocks[:2] = 1
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    pal[1] = rng.integers(0, 3, (16, 16, 16))
+    mats = np.zeros((256, 6), np.float32)
+    mats[:, :3] = rng.random((256, 3))
+    w = world(blocks, pal, mats)
+    f = field_of(4, 4, 4, rng)
+    f[..., :3] *= 3.0  # old values above 1 are clamped by store_probe
+    a1, a2 = 0.1, 0.6
+    n1 = orc.radiance_update(w, f, rays=64, time_seed=4, alpha=a1)
+    n2 = orc.radiance_update(w, f, rays=64, time_seed=4, alpha=a2)
+    old = np.clip(f[..., :3], 0, 1)
+    m1 = (n1[..., :3] - (1 - a1) * old) / a1
+    m2 = (n2[..., :3] - (1 - a2) * old) / a2
+    assert np.allclose(m1, m2, atol=2e-5)
+    assert np.all(n1[..., 3] == 1)
+
+
+@pytest.mark.parametrize("rays", [32, 64, 128])
+def test_uniform_field_fixed_point(rays):
+    """Solid world of one material (colour c, emission e), uniform field L:
+    every ray hits at its first sample and returns L*c + c*e, so
+    new = (1-a)*L + a*(L*c + c*e)."""
+    blocks, pal = solid_world(4, 4, 3, voxel=11)
+    mats = np.zeros((256, 6), np.float32)
+    c = np.float32([0.8, 0.5, 0.25])
+    e = np.float32(0.75)
+    mats[11, :3] = c
+    mats[11, 4] = e
+    L = np.float32([0.4, 0.6, 0.9])
+    f = field_of(4, 4, 3, value=L)
+    a = 0.2
+    new = orc.radiance_update(world(blocks, pal, mats), f, rays=rays, time_seed=1, alpha=a)
+    expect = (1 - a) * L + a * (L * c + c * e)
+    assert np.allclose(new[..., :3], expect, rtol=1e-6, atol=1e-6)
+
+
+def test_request_indexing_size_shift_and_untouched_probes():
+    rng = np.random.default_rng(8)
+    blocks = np.zeros((3, 3, 3), np.int16)
+    pal = np.zeros((1, 16, 16, 16), np.uint8)
+    f = field_of(3, 3, 3, rng)
+    req = np.array([[x, y, z, 0] for z in range(3) for y in range(3) for x in range(3)], np.int32)
+    new = orc.radiance_update(world(blocks, pal), f, requests=req, rays=32, size=4, shift=3, alpha=1.0)
+    touched = req[3::4][: len(req) // 4]
+    mask = np.zeros((3, 3, 3), bool)
+    mask[touched[:, 2], touched[:, 1], touched[:, 0]] = True
+    assert np.array_equal(new[~mask], f[~mask])
+    # alpha = 1 in an empty world: the new value is the mean sky/sun light
+    assert np.all(new[mask][:, :3] >= SKY - 1e-6)
+
+
+# ---------------------------------------------------------------- map.comp + host placement
+def placement(rot, shift, size):
+    from lum_amd.api import placement as pl
+    return pl(rot, shift, size)
+
+
+def map_into(world_size, blocks, static, model, p, slots=None):
+    wz, wy, wx = blocks.shape
+    pal = np.zeros(((slots or (wx * wy * wz + static)), 16, 16, 16), np.uint8)
+    orc.map_model(world_size, blocks, pal, static, model, list(p.minv), list(p.shift)[:3], list(p.groups))
+    return pal
+
+
+def world_voxels(pal, blocks):
+    """Dense voxel grid of a world (block -> palette slot)."""
+    wz, wy, wx = blocks.shape
+    out = np.zeros((wz * 16, wy * 16, wx * 16), np.uint8)
+    for z in range(wz):
+        for y in range(wy):
+            for x in range(wx):
+                out[16 * z:16 * z + 16, 16 * y:16 * y + 16, 16 * x:16 * x + 16] = pal[blocks[z, y, x]]
+    return out
+
+
+def dynamic_world(wx, wy, wz, static):
+    ids = np.arange(wx * wy * wz, dtype=np.int64).reshape(wz, wy, wx) + static
+    return ids.astype(np.int16)
+
+
+def test_placement_identity_and_translation():
+    p = placement((1, 0, 0, 0), (16.0, 3.0, 5.0), (8, 9, 10))
+    m = np.array(p.minv, np.float64).reshape(4, 4).T  # rows
+    assert np.allclose(m[:3, :3], np.eye(3))
+    assert np.allclose(m[:3, 3], [-16, -3, -5])
+    assert list(p.shift)[:3] == [16, 3, 5]
+    # area = size - 1, dispatch (area + 3) / 4 groups
+    assert list(p.groups) == [(7 + 3) // 4, (8 + 3) // 4, (9 + 3) // 4]
+    assert np.allclose(list(p.aabb_min), [16, 3, 5]) and np.allclose(list(p.aabb_max), [23, 11, 14])
+
+
+def test_map_identity_copies_model_verbatim_with_coverage_quirk():
+    rng = np.random.default_rng(4)
+    static = 4
+    blocks = dynamic_world(3, 3, 3, static)
+    model = rng.integers(1, 256, size=(18, 17, 19)).astype(np.uint8)  # z, y, x sizes
+    p = placement((1, 0, 0, 0), (5.0, 6.0, 7.0), (19, 17, 18))
+    pal = map_into((3, 3, 3), blocks, static, model, p)
+    vox = world_voxels(pal, blocks)
+    got = vox[7:7 + 18, 6:6 + 17, 5:5 + 19]
+    # the dispatch covers [min, min + 4*ceil((size-1)/4)): with size-1 a
+    # multiple of 4 (y: 16) the last layer is never visited
+    expect = model.copy()
+    expect[:, 16, :] = 0
+    assert np.array_equal(got, expect)
+    # nothing outside the model box
+    vox[7:7 + 18, 6:6 + 17, 5:5 + 19] = 0
+    assert vox.sum() == 0
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_map_rotation_about_z_is_exact_permutation(k):
+    """Quarter turns: every float operation is exact, so the mapping is the
+    closed form m = trunc(R^T (w + 0.5 - t))."""
+    rng = np.random.default_rng(10 + k)
+    static = 2
+    size = (12, 9, 7)  # x, y, z
+    model = rng.integers(1, 256, size=(size[2], size[1], size[0])).astype(np.uint8)
+    ang = k * math.pi / 2
+    q = (math.cos(ang / 2), 0.0, 0.0, math.sin(ang / 2))
+    t = (40.0, 41.0, 20.0)
+    p = placement(q, t, size)
+    blocks = dynamic_world(6, 6, 3, static)
+    pal = map_into((6, 6, 3), blocks, static, model, p)
+    vox = world_voxels(pal, blocks)
+    R = np.round(np.array([[math.cos(ang), -math.sin(ang), 0], [math.sin(ang), math.cos(ang), 0], [0, 0, 1]]))
+    expect = np.zeros_like(vox)
+    nx, ny, nz = (g * 4 for g in p.groups)
+    sx, sy, sz = list(p.shift)[:3]
+    for gz in range(nz):
+        for gy in range(ny):
+            for gx in range(nx):
+                wv = np.array([gx + sx, gy + sy, gz + sz], np.float64) + 0.5
+                mv = R.T @ (wv - np.array(t))
+                mi = np.trunc(mv).astype(int)
+                if np.all(mi >= 0) and np.all(mi < size):
+                    val = model[mi[2], mi[1], mi[0]]
+                    if val:
+                        iw = (wv - 0.5).astype(int)
+                        expect[iw[2], iw[1], iw[0]] = val
+    assert np.array_equal(vox, expect)
+
+
+def test_map_later_model_wins_and_static_blocks_untouched():
+    static = 3
+    blocks = dynamic_world(2, 2, 2, static)
+    blocks[0, 0, 0] = 1  # a static block inside the model box
+    a = np.full((8, 8, 8), 10, np.uint8)
+    b = np.zeros((8, 8, 8), np.uint8)
+    b[::2] = 20
+    pa = placement((1, 0, 0, 0), (12.0, 12.0, 12.0), (8, 8, 8))
+    pal = np.zeros((8 + static, 16, 16, 16), np.uint8)
+    for m in (a, b):
+        orc.map_model((2, 2, 2), blocks, pal, static, m, list(pa.minv), list(pa.shift)[:3], list(pa.groups))
+    vox = world_voxels(pal, blocks)
+    region = vox[12:20, 12:20, 12:20]
+    # coverage 4*ceil(7/4) = 8 voxels per axis: the whole model; b's nonzero
+    # voxels (even z layers) overwrite a, b's zeros do not; the part inside
+    # static block (0,0,0) (all coordinates < 16) is never written
+    zz, yy, xx = np.meshgrid(np.arange(12, 20), np.arange(12, 20), np.arange(12, 20), indexing="ij")
+    expect = np.where((zz - 12) % 2 == 0, 20, 10)
+    expect[(zz < 16) & (yy < 16) & (xx < 16)] = 0
+    assert np.array_equal(region, expect)
+    assert pal[1].sum() == 0  # static block never written
+
+
+def test_blockify_allocates_in_zyx_order_with_margin():
+    static = 5
+    world_b = np.zeros((4, 4, 4), np.int16)
+    world_b[1, 1, 1] = 3
+    copy_src = np.zeros(64 + static, np.int32)
+    # model AABB [16, 31]^3 -> blocks ((16-1)/16 .. (31+1)/16) = 0..2 per axis
+    counter = orc.blockify((4, 4, 4), world_b, static, static, [16, 16, 16], [31, 31, 31], copy_src)
+    assert counter == static + 27
+    ids = world_b[0:3, 0:3, 0:3].ravel()  # z, y, x order = allocation order
+    assert np.array_equal(ids, np.arange(static, static + 27))
+    assert copy_src[static + 13] == 3 and copy_src[static:static + 27].sum() == 3
+    # a second model over the same blocks reuses them
+    assert orc.blockify((4, 4, 4), world_b, static, counter, [16, 16, 16], [20, 20, 20], copy_src) == counter
+
+
+def test_exec_copies_fills_allocated_slots():
+    pal = np.zeros((10, 16, 16, 16), np.uint8)
+    pal[2] = 9
+    pal[7] = 99  # stale content of a slot beyond the counter
+    copy_src = np.zeros(10, np.int32)
+    copy_src[4] = 2
+    orc.exec_copies(pal, 3, 6, copy_src)
+    assert np.all(pal[4] == 9) and pal[3].sum() == 0 and pal[5].sum() == 0 and pal[7].sum() == 0
+
+
+# ---------------------------------------------------------------- probe selection
+def test_select_probes_neighbourhood_and_specials():
+    b = np.zeros((4, 5, 6), np.int16)
+    b[2, 3, 1] = 1
+    got = orc.select_probes(b, np.array([[5, 0, 0, 0], [1, 3, 2, 0]], np.int32))
+    sel = {(x, y, z) for z in range(1, 4) for y in range(2, 5) for x in range(0, 3)}
+    coords = [tuple(r[:3]) for r in got]
+    assert set(coords[:-1]) == sel and coords[-1] == (5, 0, 0)
+    # x-fastest order
+    keys = [x + 6 * (y + 5 * z) for x, y, z in coords[:-1]]
+    assert keys == sorted(keys)
+
+
+# ---------------------------------------------------------------- glossy
+def test_glossy_sky_closed_form_and_mask():
+    from lum_amd.api import camera_default
+    blocks = np.zeros((4, 4, 4), np.int16)
+    pal = np.zeros((1, 16, 16, 16), np.uint8)
+    mats = np.zeros((256, 6), np.float32)
+    mats[1] = [0.5, 0.25, 1.0, 1.0, 0.5, 0.2]   # glossy
+    mats[2] = [1.0, 1.0, 1.0, 1.0, 0.0, 0.9]   # rough: masked
+    F = np.float32([0.3, 0.2, 0.1])
+    f = field_of(4, 4, 4, value=F)
+    cam = camera_default(8, 4)
+    ubo = np.array(list(cam.campos) + list(cam.camdir) + list(cam.horiz_scaled) + list(cam.verti_scaled)
+                   + list(cam.light_dir) + list(cam.frame_size), np.float32)
+    mn = np.zeros((4, 8, 4), np.uint8)
+    mn[..., 0] = 1
+    mn[..., 1:] = 255  # normal decodes to (1, 1, 1), not renormalised
+    mn[1, 3, 0] = 2
+    dp = np.full((4, 8), 0.05, np.float32)
+    out, keys = orc.glossy(world(blocks, pal, mats), f, mn, dp, ubo)
+    c, e = mats[1, :3], mats[1, 4]
+    L0 = c * (e + F)
+    d = np.array(cam.camdir, np.float64)
+    n = np.ones(3)
+    r = d - 2 * np.dot(n, d) * n
+    s = -np.dot(r, np.array(cam.light_dir))
+    sun = np.float32([0.9, 0.9, 0.6]) * 0.5 * c * (s - 0.9) * 10 if s > 0.9 else 0
+    expect = (L0 + sun + SKY * c) / 8
+    m = np.ones((4, 8), bool)
+    m[1, 3] = False
+    assert np.allclose(out[m][:, :3], expect, rtol=1e-5, atol=1e-7)
+    assert np.allclose(out[m][:, 3], 1 - mats[1, 5])
+    assert np.all(out[1, 3] == 0) and np.all(keys == 0)
+
+
+def test_camera_default_matches_lum_camera():
+    from lum_amd.api import camera_default
+    cam = camera_default(1920, 1080)
+    fd = np.float32([0.61, 1.0, -0.8])
+    fd = fd / np.float32(np.sqrt(np.float32(fd @ fd)))
+    d = fd.astype(np.float64)
+    plane = np.array([d[0], d[1], 0.0])
+    plane /= np.linalg.norm(plane)
+    horiz = np.cross(plane, [0, 0, 1.0])
+    horiz /= np.linalg.norm(horiz)
+    verti = np.cross(d, horiz)
+    verti /= np.linalg.norm(verti)
+    assert np.allclose(cam.campos, [60, 0, 194])
+    assert np.allclose(cam.camdir, d, atol=1e-7)
+    assert np.allclose(cam.horiz_scaled, horiz * 384 / 2, atol=1e-4)
+    assert np.allclose(cam.verti_scaled, verti * 216 / 2, atol=1e-4)
+    assert np.allclose(list(cam.frame_size), [1920, 1080])
+    L = np.array([0.5, 0.5, -0.9])
+    assert np.allclose(cam.light_dir, L / np.linalg.norm(L), atol=1e-7)
