@@ -504,12 +504,168 @@ def cpu_baseline_serving(trace, hosts, perm_host, sizes, seconds=10.0):
                       f"C++ oracle on {threads} threads"}
 
 
+def run_chain(args):
+    """the 48x48x16 world plus 32 models of 16^3-64^3
+    (30% fill, uniform rotations) mapped, every probe updated (64 rays), and a
+    1920x1080 glossy march (max 128 steps) from a synthetic G-buffer, FP32.
+    A step is the whole map -> radiance -> glossy chain on one stream."""
+    import torch
+    from lum_amd import synth
+    from lum_amd.api import Context, World, Model, placement, camera_default, radiance_params
+
+    cfg = synth.PLAIN
+    rng = np.random.default_rng(cfg["seed"])
+    size = cfg["size"]
+    blocks = synth.make_world(size, rng, cfg["n_palette"])
+    pal = synth.make_palette(rng, cfg["n_palette"])
+    mats = synth.make_materials(rng)
+    field = synth.make_field(size, rng)
+    vox, quats, shifts = synth.make_models(rng, n=32, lo=16, hi=64, fill=0.3, world_size=size)
+    W, H = 1920, 1080
+    mn, dp = synth.make_gbuffer(W, H, mats, rng)
+    ctx = Context(0)
+    w = World(ctx, size, static_blocks=cfg["n_palette"])
+    w.set_blocks(blocks)
+    w.set_palette(pal)
+    w.set_materials(mats)
+    w.write_radiance(field)
+    ms = [Model(ctx, v) for v in vox]
+    pls = [placement(q, s_, m.size) for q, s_, m in zip(quats, shifts, ms)]
+    mn_d, dp_d = torch.from_numpy(mn).cuda(), torch.from_numpy(dp).cuda()
+    cam = camera_default(W, H)
+    out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    map_threads = sum(p.groups[0] * p.groups[1] * p.groups[2] * 64 for p in pls)
+
+    def step(k):
+        w.map_models_placed(ms, pls)
+        w.radiance_update(None, radiance_params(rays=64, time_seed=k, alpha=0.1, max_steps=64))
+        w.glossy(mn_d, dp_d, cam, 128.0, 128, out=out)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    pass_ms = {"map": [], "radiance": [], "glossy": []}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(args.steps):
+        step(100 + k)
+        torch.cuda.synchronize()
+        pass_ms["map"].append(w.last_pass_ms(1))
+        pass_ms["radiance"].append(w.last_pass_ms(0))
+        pass_ms["glossy"].append(w.last_pass_ms(2))
+    e1.record()
+    torch.cuda.synchronize()
+    total = e0.elapsed_time(e1)
+    per = {k: float(np.mean(v)) for k, v in pass_ms.items()}
+    rays = w.n_blocks * 64
+    glossy_px = int((mats[mn[..., 0], 5] <= 0.5).sum())
+    res = {
+        "metric": "probe-rays/s (map -> radiance -> glossy chain, 48x48x16 world, 32 models, 1920x1080 glossy)",
+        "value": rays / (per["radiance"] / 1e3),
+        "unit": "probe-rays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": total / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded world, palette, materials, 32 models, G-buffer)",
+        "config": {"workload": "chain: 48x48x16 world + 32 models 16^3-64^3 at 30% fill (uniform rotations), "
+                               "36864 probes x 64 rays, glossy 1920x1080 max 128 steps, FP32",
+                   "palette_slots_used": w.palette_counter(), "map_threads": map_threads,
+                   "glossy_pixels": glossy_px},
+        "pass_ms": per,
+        "map_voxels_per_s": map_threads / (per["map"] / 1e3),
+        "glossy_rays_per_s": glossy_px / (per["glossy"] / 1e3),
+    }
+    for m in ms:
+        m.close()
+    w.close()
+    ctx.close()
+    return res
+
+
+def run_precision(args):
+    """a 256x256x32-block world (2,097,152 probes),
+    128 rays per probe, max 96 steps, 8 accumulated frames, radiance stored in
+    FP32, FP16 and BF16 (FP32 march and accumulation).  Each is compared with
+    the FP64 CPU oracle run on the same inputs, on a seeded 4,096-probe
+    subset."""
+    import torch
+    import tests._oracle as orc
+    from lum_amd import synth
+    from lum_amd.api import Context, World, radiance_params
+    from lum_amd._lib import RADIANCE_F32, RADIANCE_F16, RADIANCE_BF16
+
+    size = tuple(int(v) for v in args.size.split(",")) if args.size else (256, 256, 32)
+    rays, steps, frames, alpha = 128, 96, args.frames, 0.1
+    rng = np.random.default_rng(31337)
+    blocks = synth.make_world(size, rng, 64)
+    pal = synth.make_palette(rng, 64)
+    mats = synth.make_materials(rng)
+    field = synth.make_field(size, rng)
+    n = size[0] * size[1] * size[2]
+    subset = np.sort(rng.choice(n, size=min(4096, n), replace=False))
+    ctx = Context(0)
+    res_fmt = {}
+    gpu_fields = {}
+    for name, fmt in (("fp32", RADIANCE_F32), ("fp16", RADIANCE_F16), ("bf16", RADIANCE_BF16)):
+        w = World(ctx, size, static_blocks=64, palette_slots=64, radiance_format=fmt)
+        w.set_blocks(blocks)
+        w.set_palette(pal)
+        w.set_materials(mats)
+        w.write_radiance(field)
+        w.upload_blocks()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for f in range(frames):
+            w.radiance_update(None, radiance_params(rays=rays, time_seed=f, alpha=alpha, max_steps=steps))
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / frames
+        gpu_fields[name] = w.read_radiance().reshape(-1, 4)[subset, :3].astype(np.float64)
+        res_fmt[name] = {"ms_per_frame": ms, "probe_rays_per_s": n * rays / (ms / 1e3)}
+        w.close()
+    ctx.close()
+    # FP64 oracle: full field (bounce lookups read every probe), all host threads
+    t0 = time.perf_counter()
+    wd = orc.WorldData(blocks, pal, mats)
+    f64 = field.astype(np.float64)
+    for f in range(frames):
+        f64 = orc.radiance_update(wd, f64, rays=rays, time_seed=f, alpha=alpha, max_steps=steps,
+                                  threads=os.cpu_count() or 1, fp64=True)
+    cpu_s = time.perf_counter() - t0
+    ref = f64.reshape(-1, 4)[subset, :3]
+    for name, got in gpu_fields.items():
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3)
+        res_fmt[name].update({"max_rel_err": float(rel.max()), "mean_rel_err": float(rel.mean()),
+                              "p99_rel_err": float(np.percentile(rel, 99)),
+                              "frac_within_1e-3": float((rel <= 1e-3).mean())})
+    fp32 = gpu_fields["fp32"]
+    for name in ("fp16", "bf16"):
+        rel = np.abs(gpu_fields[name] - fp32) / np.maximum(np.abs(fp32), 1e-3)
+        res_fmt[name]["max_rel_err_vs_fp32_storage"] = float(rel.max())
+        res_fmt[name]["mean_rel_err_vs_fp32_storage"] = float(rel.mean())
+    return {
+        "metric": "radiance error of FP32/FP16/BF16 storage vs an FP64 CPU oracle (and probe-rays/s)",
+        "value": res_fmt["fp16"]["probe_rays_per_s"],
+        "unit": "probe-rays/s",
+        "n_gpus": 1, "steps": frames, "warmup": 0,
+        "ms_per_step": res_fmt["fp16"]["ms_per_frame"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (fp16 storage)",
+        "data": "synthetic (seeded world, palette, materials, field)",
+        "config": {"workload": "precision: %dx%dx%d blocks, %d rays, max %d steps, %d frames, alpha %.1f; "
+                               "error on a seeded %d-probe subset" % (size + (rays, steps, frames, alpha, len(subset)))},
+        "formats": res_fmt,
+        "fp64_oracle_cpu_s": cpu_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="serving", choices=["serving", "plain", "batched"])
+    ap.add_argument("--workload", default="serving", choices=["serving", "plain", "batched", "chain", "precision"])
+    ap.add_argument("--frames", type=int, default=8)
     ap.add_argument("--serving-worlds", type=int, default=200)
     ap.add_argument("--requests", type=int, default=4000)
     ap.add_argument("--max-batch", type=int, default=64)
@@ -531,6 +687,10 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         out = run_batched(args)
+    elif args.workload == "chain":
+        out = run_chain(args)
+    elif args.workload == "precision":
+        out = run_precision(args)
     elif args.workload == "plain":
         out = run_plain(args)
     else:
