@@ -117,6 +117,11 @@ void lumx_lum_request_radiance_update(LumRenderer instance, int x, int y, int z)
  * matnorm u8x4, depth = view depth / 1000, out float4); NULL detaches. */
 void lumx_lum_set_gbuffer(LumRenderer instance, const uint8_t* matnorm_dev, const float* depth_dev, int width,
                           int height, float* out_dev);
+/* Renderer::setStoredRadianceShift / getStoredRadianceShift
+ * (renderer.hpp:125-127, renderer.cpp:604-609): the shift applied to the
+ * radiance field at every lum_submit_frame, before its radiance update. */
+void lumx_lum_set_radiance_shift(LumRenderer instance, int x, int y, int z);
+void lumx_lum_get_radiance_shift(LumRenderer instance, int* out3);
 /* The pass-level world handle (lumx_world_t of lumx.h). */
 void* lumx_lum_world(LumRenderer instance);
 

@@ -29,10 +29,16 @@
  *                          (src/renderer/src/internal_render.cpp:977-1097,1375-1527)
  *   lumx_radiance_update*  update_radiance -> shaders/radiance.comp
  *                          (src/renderer/src/internal_render.cpp:1129-1214)
- *   lumx_select_probes     update_radiance probe selection
+ *   lumx_select_probes*    update_radiance probe selection
  *                          (src/renderer/src/internal_render.cpp:1142-1168)
+ *   lumx_world_shift_radiance shift_radiance, applied by Lum::Renderer with
+ *                          setStoredRadianceShift's value
+ *                          (src/renderer/src/internal_render.cpp:1218-1335,
+ *                          src/renderer/api/renderer.cpp:436,604-609)
  *   lumx_glossy            glossy_raygen stencil + glossy -> shaders/glossy.frag
  *                          (src/renderer/src/internal_render.cpp:1990-1998,2027-2037)
+ *   lumx_diffuse           diffuse -> shaders/diffuse.frag
+ *                          (src/renderer/src/internal_render.cpp:1968-1977)
  *   lumx_camera_default    Camera::updateCamera + start_raygen UBO fill
  *                          (src/renderer/src/internal_render.cpp:923-934,1580-1592)
  */
@@ -109,6 +115,15 @@ typedef struct {
     float light_dir[3];     /* globalLightDir */
     float frame_size[2];    /* framebuffer size in pixels */
 } lumx_camera;
+
+/* Shadow map of the diffuse pass's sun term: the reference's D16_UNORM
+ * lightmap (1024 x 1024, internal_render.cpp:71,119-126; internal_render.hpp:450)
+ * and ubo.lightmap_proj. */
+typedef struct {
+    const uint16_t* depth_dev; /* row-major width x height D16 values; NULL = no sun term */
+    int width, height;
+    float proj[16];            /* column-major mat4 */
+} lumx_lightmap;
 
 /* Launch shape of the radiance update.  No reference counterpart: it
  * replaces the fixed 64-lane workgroup of radiance.comp:37,360. */
@@ -198,15 +213,38 @@ int lumx_world_get_tuning(lumx_world_t w, lumx_radiance_tuning* out);
  * counterpart. */
 int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int reps,
                            lumx_radiance_tuning* best, float* best_ms);
-/* Probe selection of update_radiance on the current (blockified) world. */
+/* Probe selection of update_radiance on the current (blockified) world.
+ * Specials outside the world are an error; a special repeated in the list
+ * is taken once. */
 int lumx_select_probes(lumx_world_t w, const int32_t* special_host, int n_special,
                        int32_t* requests_host, int* n_out);
+/* The same selection on the device, stream-ordered after the last map:
+ * requests_dev (room for n_blocks int32x4 entries: a probe appears at most
+ * once) receives the list, *count_dev its length.  Specials outside the world
+ * are skipped. */
+int lumx_select_probes_device(lumx_world_t w, const int32_t* special_dev, int n_special,
+                              int32_t* requests_dev, int32_t* count_dev, void* stream);
+/* lumx_radiance_update over a list whose length is read on the device
+ * (*count_dev, at most max_requests), e.g. the output of
+ * lumx_select_probes_device: no host round trip between the two. */
+int lumx_radiance_update_counted(lumx_world_t w, const int32_t* requests_dev, const int32_t* count_dev,
+                                 int max_requests, const lumx_radiance_params* p, void* stream);
+/* shift_radiance (src/renderer/src/internal_render.cpp:1218-1335): the
+ * texel at p + (dx, dy, dz) moves to p wherever both lie in the world;
+ * other texels keep their value.  A shift of a whole world size or more on
+ * any axis changes nothing.  The shifted field becomes current. */
+int lumx_world_shift_radiance(lumx_world_t w, int dx, int dy, int dz, void* stream);
 
 /* glossy.frag over a width x height G-buffer (matnorm u8x4, depth = view
  * depth / 1000).  Pixels whose material roughness > 0.5 are not written. */
 int lumx_glossy(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_dev, int width,
                 int height, const lumx_camera* cam, float max_dist, int max_steps, float* out_dev,
                 int32_t* hit_keys_dev, void* stream);
+/* diffuse.frag (the radiance field's consumer) over a width x height
+ * G-buffer: out_dev float4 per pixel = ((2 * field sample at origin + 6 *
+ * normal + emission + sun) * colour / 8, 1).  lightmap may be NULL. */
+int lumx_diffuse(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_dev, int width, int height,
+                 const lumx_camera* cam, const lumx_lightmap* lightmap, float* out_dev, void* stream);
 /* Lum's default camera (internal_types.hpp:54-66) for a frame size. */
 int lumx_camera_default(int frame_w, int frame_h, lumx_camera* out);
 

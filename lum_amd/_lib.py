@@ -58,6 +58,10 @@ class Camera(C.Structure):
                 ("verti_scaled", C.c_float * 3), ("light_dir", C.c_float * 3), ("frame_size", C.c_float * 2)]
 
 
+class Lightmap(C.Structure):
+    _fields_ = [("depth_dev", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("proj", C.c_float * 16)]
+
+
 class ServerDesc(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("streams", C.c_int), ("reserved", C.c_int)]
 
@@ -112,7 +116,11 @@ _SIGS = {
     "lumx_radiance_update": (_I, [_P, _P, _I, C.POINTER(RadianceParams), _P]),
     "lumx_radiance_update_traced": (_I, [_P, _P, _I, C.POINTER(RadianceParams), _P, _P, _P]),
     "lumx_select_probes": (_I, [_P, _P, _I, _P, C.POINTER(C.c_int)]),
+    "lumx_select_probes_device": (_I, [_P, _P, _I, _P, _P, _P]),
+    "lumx_radiance_update_counted": (_I, [_P, _P, _P, _I, C.POINTER(RadianceParams), _P]),
+    "lumx_world_shift_radiance": (_I, [_P, _I, _I, _I, _P]),
     "lumx_glossy": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.c_float, _I, _P, _P, _P]),
+    "lumx_diffuse": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.POINTER(Lightmap), _P, _P]),
     "lumx_camera_default": (_I, [_I, _I, C.POINTER(Camera)]),
     "lumx_world_last_pass_ms": (_I, [_P, _I, C.POINTER(C.c_float)]),
     "lumx_world_set_tuning": (_I, [_P, C.POINTER(Tuning)]),

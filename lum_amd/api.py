@@ -201,6 +201,10 @@ class World:
         import torch
         p = params if params is not None else radiance_params(**kw)
         n = 0 if requests is None else int(requests.shape[0])
+        if requests is not None and n == 0:
+            # An empty list is not "every probe": torch reports a null data
+            # pointer for empty tensors, so pass a real one with n = 0.
+            requests = torch.zeros((1, 4), dtype=torch.int32, device=requests.device)
         if not traced:
             check(self.lib.lumx_radiance_update(self.handle, _ptr(requests), n, C.byref(p), _stream()))
             return None
@@ -219,6 +223,28 @@ class World:
         check(self.lib.lumx_select_probes(self.handle, _np_ptr(sp), len(sp), _np_ptr(out), C.byref(n)))
         return out[: n.value]
 
+    def select_probes_device(self, special=None):
+        """Device selection: returns (requests int32 (n_blocks, 4), count int32
+        (1,)), both torch tensors on the device; rows past count[0] are
+        unspecified."""
+        import torch
+        n_sp = 0 if special is None else int(special.shape[0])
+        out = torch.empty((self.n_blocks, 4), dtype=torch.int32, device="cuda")
+        count = torch.empty(1, dtype=torch.int32, device="cuda")
+        check(self.lib.lumx_select_probes_device(self.handle, _ptr(special), n_sp, _ptr(out), _ptr(count),
+                                                 _stream()))
+        return out, count
+
+    def radiance_update_counted(self, requests, count, params=None, **kw):
+        """Radiance update over requests[:count[0]], the count read on the device."""
+        p = params if params is not None else radiance_params(**kw)
+        check(self.lib.lumx_radiance_update_counted(self.handle, _ptr(requests), _ptr(count),
+                                                    int(requests.shape[0]), C.byref(p), _stream()))
+
+    def shift_radiance(self, shift):
+        check(self.lib.lumx_world_shift_radiance(self.handle, int(shift[0]), int(shift[1]), int(shift[2]),
+                                                 _stream()))
+
     def glossy(self, matnorm, depth, cam, max_dist=128.0, max_steps=0, out=None, traced=False):
         """matnorm: torch uint8 (h, w, 4); depth: torch float32 (h, w); returns float32 (h, w, 4)."""
         import torch
@@ -229,6 +255,24 @@ class World:
         check(self.lib.lumx_glossy(self.handle, _ptr(matnorm), _ptr(depth), w, h, C.byref(cam),
                                    float(max_dist), int(max_steps), _ptr(out), _ptr(keys), _stream()))
         return (out, keys) if traced else out
+
+
+    def diffuse(self, matnorm, depth, cam, lightmap=None, lightmap_proj=None, out=None):
+        """diffuse.frag: matnorm torch uint8 (h, w, 4), depth float32 (h, w);
+        lightmap: torch int16/uint16 (lh, lw) D16 values or None; lightmap_proj:
+        16 floats, column-major.  Returns float32 (h, w, 4)."""
+        import torch
+        h, w = int(depth.shape[0]), int(depth.shape[1])
+        if out is None:
+            out = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+        lm = None
+        if lightmap is not None:
+            proj = np.ascontiguousarray(lightmap_proj, np.float32).reshape(16)
+            lm = _lib.Lightmap(lightmap.data_ptr(), int(lightmap.shape[1]), int(lightmap.shape[0]),
+                               (C.c_float * 16)(*proj))
+        check(self.lib.lumx_diffuse(self.handle, _ptr(matnorm), _ptr(depth), w, h, C.byref(cam),
+                                    None if lm is None else C.byref(lm), _ptr(out), _stream()))
+        return out
 
 
 class Batch:
@@ -300,6 +344,9 @@ class Graph:
 
 
 def radiance_request(world_id, params, requests=None):
+    """requests must stay alive until the request completes."""
+    if requests is not None and int(requests.shape[0]) == 0:
+        raise ValueError("an empty request list updates nothing; pass None for every probe")
     r = Request()
     r.kind = REQ_RADIANCE
     r.world = world_id

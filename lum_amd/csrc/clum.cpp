@@ -45,8 +45,12 @@ struct Renderer {
     int frame = 0;
     int magic_size = 2;
     float radiance_ms = 0.0f;
-    int32_t* req_dev = nullptr;
-    std::vector<int32_t> req_host;
+    int32_t* req_dev = nullptr;      // probe list of update_radiance (n_blocks entries)
+    int32_t* count_dev = nullptr;    // its length, written by the device selection
+    int32_t* special_dev = nullptr;  // specials of the frame being submitted
+    size_t special_cap = 0;          // entries special_dev holds
+    std::vector<int32_t> special_up; // host source of special_dev until the frame ends
+    int radiance_shift[3] = {0, 0, 0};  // Renderer::stored_radiance_shift
     const uint8_t* gb_matnorm = nullptr;
     const float* gb_depth = nullptr;
     float* gb_out = nullptr;
@@ -213,6 +217,11 @@ static void release(Renderer* r) {
     r->world = nullptr;
     if (r->req_dev) hipFree(r->req_dev);
     r->req_dev = nullptr;
+    if (r->count_dev) hipFree(r->count_dev);
+    r->count_dev = nullptr;
+    if (r->special_dev) hipFree(r->special_dev);
+    r->special_dev = nullptr;
+    r->special_cap = 0;
     if (r->ctx) lumx_context_destroy(r->ctx);
     r->ctx = nullptr;
 }
@@ -246,8 +255,9 @@ void lum_init(LumRenderer instance, const LumSettings* settings) {
     if (lumx_world_create(r->ctx, &d, &r->world) != LUMX_OK) return;
     r->blocks.assign(settings->static_block_palette_size, std::array<uint8_t, 4096>{});
     size_t nb = (size_t)d.size[0] * d.size[1] * d.size[2];
+    // Every probe appears at most once in the selection: n_blocks entries suffice.
     if (hipMalloc(&r->req_dev, nb * 16) != hipSuccess) r->req_dev = nullptr;
-    r->req_host.resize(nb * 4);
+    if (hipMalloc(&r->count_dev, sizeof(int32_t)) != hipSuccess) r->count_dev = nullptr;
 }
 
 void lum_load_world(LumRenderer instance, const LumMeshBlock* world_data) {
@@ -355,7 +365,7 @@ void lum_prepare_frame(LumRenderer instance) {
 
 void lum_submit_frame(LumRenderer instance) {
     Renderer* r = get(instance);
-    if (!r || !r->world || !r->req_dev) return;
+    if (!r || !r->world || !r->req_dev || !r->count_dev) return;
     update_camera(r->cam);
     void* stream = lumx_context_stream(r->ctx);
     // blockify + exec_copies + map.comp
@@ -371,12 +381,31 @@ void lum_submit_frame(LumRenderer instance) {
     if (lumx_map_models(r->world, (int)models.size(), models.data(), trans.data(), stream) != LUMX_OK) return;
     float map_ms = 0.0f;
     lumx_world_last_pass_ms(r->world, 1, &map_ms);
-    // update_radiance: probe selection, work-size controller, dispatch.
-    int n = 0;
-    if (lumx_select_probes(r->world, r->special.data(), (int)(r->special.size() / 4), r->req_host.data(), &n) !=
+    // shift_radiance with the stored shift (renderer.cpp:436), then
+    // update_radiance: probe selection on the device, work-size controller,
+    // dispatch over the device-side count (no host round trip).
+    if (lumx_world_shift_radiance(r->world, r->radiance_shift[0], r->radiance_shift[1], r->radiance_shift[2],
+                                  stream) != LUMX_OK)
+        return;
+    r->special_up.swap(r->special);
+    r->special.clear();
+    const size_t n_special = r->special_up.size() / 4;
+    if (n_special > r->special_cap) {
+        if (r->special_dev) hipFree(r->special_dev);
+        r->special_cap = 0;
+        if (hipMalloc(&r->special_dev, n_special * 16) != hipSuccess) {
+            r->special_dev = nullptr;
+            return;
+        }
+        r->special_cap = n_special;
+    }
+    // special_up stays untouched until the stream synchronisation below.
+    if (n_special && hipMemcpyAsync(r->special_dev, r->special_up.data(), n_special * 16, hipMemcpyHostToDevice,
+                                    (hipStream_t)stream) != hipSuccess)
+        return;
+    if (lumx_select_probes_device(r->world, r->special_dev, (int)n_special, r->req_dev, r->count_dev, stream) !=
         LUMX_OK)
         return;
-    r->special.clear();
     if (r->radiance_ms < 1.8f) r->magic_size--;
     else if (r->radiance_ms > 2.2f) r->magic_size++;
     r->magic_size = std::min(std::max(r->magic_size, 1), 10);
@@ -388,13 +417,11 @@ void lum_submit_frame(LumRenderer instance) {
     p.alpha = 0.01f;  // REACTIVNESS default (radiance.comp:28)
     p.max_dist = 128.0f;
     p.max_steps = 0;
-    if (n > 0) {
-        if (hipMemcpyAsync(r->req_dev, r->req_host.data(), (size_t)n * 16, hipMemcpyHostToDevice,
-                           (hipStream_t)stream) != hipSuccess)
-            return;
-        if (lumx_radiance_update(r->world, r->req_dev, n, &p, stream) != LUMX_OK) return;
-        lumx_world_last_pass_ms(r->world, 0, &r->radiance_ms);
-    }
+    lumx_world_desc wd;
+    lumx_world_info(r->world, &wd);
+    const int n_blocks = wd.size[0] * wd.size[1] * wd.size[2];
+    if (lumx_radiance_update_counted(r->world, r->req_dev, r->count_dev, n_blocks, &p, stream) != LUMX_OK) return;
+    lumx_world_last_pass_ms(r->world, 0, &r->radiance_ms);
     float glossy_ms = 0.0f;
     if (r->gb_matnorm && r->gb_depth && r->gb_out) {
         lumx_camera cam;
@@ -406,7 +433,7 @@ void lum_submit_frame(LumRenderer instance) {
     hipStreamSynchronize((hipStream_t)stream);
     r->ts[0] = 0.0;
     r->ts[1] = map_ms;
-    r->ts[2] = r->ts[1] + (n > 0 ? r->radiance_ms : 0.0f);
+    r->ts[2] = r->ts[1] + r->radiance_ms;
     r->ts[3] = r->ts[2] + glossy_ms;
     r->frame++;
 }
@@ -478,6 +505,20 @@ void lumx_lum_set_gbuffer(LumRenderer instance, const uint8_t* matnorm_dev, cons
     r->gb_out = out_dev;
     r->gb_w = width;
     r->gb_h = height;
+}
+
+void lumx_lum_set_radiance_shift(LumRenderer instance, int x, int y, int z) {
+    Renderer* r = get(instance);
+    if (!r) return;
+    r->radiance_shift[0] = x;
+    r->radiance_shift[1] = y;
+    r->radiance_shift[2] = z;
+}
+
+void lumx_lum_get_radiance_shift(LumRenderer instance, int* out3) {
+    Renderer* r = get(instance);
+    if (!r || !out3) return;
+    for (int k = 0; k < 3; ++k) out3[k] = r->radiance_shift[k];
 }
 
 void* lumx_lum_world(LumRenderer instance) {

@@ -285,7 +285,12 @@ static hipError_t world_alloc_device(lumx_world* w) {
     chk(hipMalloc(&w->empty_bits, ((nb + 31) / 32) * sizeof(uint32_t)));
     chk(hipMalloc(&w->pal_bits, slots * 128 * sizeof(uint32_t)));
     chk(hipMalloc(&w->batch_counter, 2 * sizeof(unsigned int)));
+    chk(hipMalloc(&w->sel_bits, lumx::select_bits_bytes(w->n_blocks)));
+    chk(hipMalloc(&w->sel_tiles, lumx::select_tiles(w->n_blocks) * sizeof(int)));
+    chk(hipMalloc(&w->sel_claim, nb * sizeof(int)));
     if (e != hipSuccess) return e;
+    static_assert(sizeof(int) == 4, "claim words");
+    chk(hipMemset(w->sel_claim, lumx::select_claim_idle() & 0xff, nb * sizeof(int)));
     chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
     chk(hipMemset(w->pal_bits, 0, slots * 128 * sizeof(uint32_t)));
     chk(hipMemset(w->batch_counter, 0, 2 * sizeof(unsigned int)));
@@ -301,7 +306,8 @@ static hipError_t world_alloc_device(lumx_world* w) {
 static size_t world_device_bytes(const lumx_world* w) {
     const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
     return nb * 2 + slots * 4096 + 256 * 6 * 4 + slots * 4 + 2 * nb * w->texel_bytes + ((nb + 31) / 32) * 4 +
-           slots * 128 * 4 + 8;
+           slots * 128 * 4 + 8 + lumx::select_bits_bytes(w->n_blocks) +
+           lumx::select_tiles(w->n_blocks) * sizeof(int) + nb * sizeof(int);
 }
 
 static void world_free_device(lumx_world* w) {
@@ -314,6 +320,12 @@ static void world_free_device(lumx_world* w) {
     hipFree(w->empty_bits);
     hipFree(w->pal_bits);
     hipFree(w->batch_counter);
+    hipFree(w->sel_bits);
+    hipFree(w->sel_tiles);
+    hipFree(w->sel_claim);
+    w->sel_bits = nullptr;
+    w->sel_tiles = nullptr;
+    w->sel_claim = nullptr;
     w->blocks = nullptr;
     w->palette = nullptr;
     w->mats = nullptr;
@@ -924,10 +936,11 @@ int lumx::radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx
 // Launch one radiance update from field[cur] into field[cur ^ 1] with tuning t.
 static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
                            int32_t* keys, int32_t* steps, bool traced, hipStream_t s,
-                           const lumx_radiance_tuning& tune) {
+                           const lumx_radiance_tuning& tune, const int32_t* count_dev = nullptr) {
     RadianceArgs a;
     int rc = radiance_args(w, req, n_req, p, &a);
     if (rc) return rc;
+    a.n_requests_dev = count_dev;
     a.ray_keys = keys;
     a.ray_steps = steps;
     // Probes outside the request list keep their value in the new field.
@@ -962,14 +975,15 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
 }
 
 static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
-                         int32_t* keys, int32_t* steps, bool traced, void* stream) {
+                         int32_t* keys, int32_t* steps, bool traced, void* stream,
+                         const int32_t* count_dev = nullptr) {
     int rc = radiance_check(w, req, n_req, p);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(w->ctx->device));
     hipStream_t s = pick_stream(w->ctx, stream);
     const bool timed = !stream_capturing(s);
     if (timed) HIP_TRY(hipEventRecord(w->ev[0][0], s));
-    rc = radiance_launch(w, req, n_req, p, keys, steps, traced, s, w->tune);
+    rc = radiance_launch(w, req, n_req, p, keys, steps, traced, s, w->tune, count_dev);
     if (rc) return rc;
     if (timed) {
         HIP_TRY(hipEventRecord(w->ev[0][1], s));
@@ -1063,6 +1077,65 @@ int lumx_radiance_update_traced(lumx_world_t w, const int32_t* req, int n, const
     return radiance_impl(w, req, n, p, keys, steps, true, stream);
 }
 
+int lumx_radiance_update_counted(lumx_world_t w, const int32_t* req, const int32_t* count_dev, int max_requests,
+                                 const lumx_radiance_params* p, void* stream) {
+    if (!req || !count_dev) return fail(LUMX_EINVAL, "counted update needs a request list and its device count");
+    return radiance_impl(w, req, max_requests, p, nullptr, nullptr, false, stream, count_dev);
+}
+
+int lumx_select_probes_device(lumx_world_t w, const int32_t* special_dev, int n_special, int32_t* out_dev,
+                              int32_t* count_dev, void* stream) {
+    if (!w || !out_dev || !count_dev || n_special < 0 || (n_special > 0 && !special_dev))
+        return fail(LUMX_EINVAL, "bad argument");
+    if (n_special >= lumx::select_claim_idle()) return fail(LUMX_EINVAL, "too many special requests");
+    REQUIRE_RESIDENT(w);
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    lumx::SelectArgs a{};
+    a.blocks = w->blocks;
+    a.wx = w->desc.size[0];
+    a.wy = w->desc.size[1];
+    a.wz = w->desc.size[2];
+    a.n = w->n_blocks;
+    a.n_tiles = (int)lumx::select_tiles(w->n_blocks);
+    a.bits = w->sel_bits;
+    a.tile_counts = w->sel_tiles;
+    a.special = (const int4*)special_dev;
+    a.n_special = n_special;
+    a.claim = w->sel_claim;
+    a.out = (int4*)out_dev;
+    a.count = count_dev;
+    HIP_TRY(lumx::launch_select(a, s));
+    return LUMX_OK;
+}
+
+int lumx_world_shift_radiance(lumx_world_t w, int dx, int dy, int dz, void* stream) {
+    if (!w) return fail(LUMX_EINVAL, "null world");
+    REQUIRE_RESIDENT(w);
+    const int d[3] = {dx, dy, dz};
+    // A shift of a whole world size or more leaves no intersection: nothing moves.
+    for (int k = 0; k < 3; ++k)
+        if (std::llabs((long long)d[k]) >= w->desc.size[k]) return LUMX_OK;
+    if (dx == 0 && dy == 0 && dz == 0) return LUMX_OK;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    lumx::ShiftArgs a{};
+    a.in = w->field[w->cur];
+    a.out = w->field[w->cur ^ 1];
+    a.wx = w->desc.size[0];
+    a.wy = w->desc.size[1];
+    a.wz = w->desc.size[2];
+    a.n = w->n_blocks;
+    for (int k = 0; k < 3; ++k) {
+        a.shift[k] = d[k];
+        a.dst[k] = d[k] >= 0 ? 0 : -d[k];
+        a.extent[k] = w->desc.size[k] - (int)std::llabs((long long)d[k]);
+    }
+    HIP_TRY(lumx::launch_shift(a, (int)w->texel_bytes, s));
+    w->cur ^= 1;
+    return LUMX_OK;
+}
+
 int lumx_select_probes(lumx_world_t w, const int32_t* special, int n_special, int32_t* out, int* n_out) {
     if (!w || !out || !n_out || n_special < 0 || (n_special > 0 && !special)) return fail(LUMX_EINVAL, "bad argument");
     const int wx = w->desc.size[0], wy = w->desc.size[1], wz = w->desc.size[2];
@@ -1153,6 +1226,43 @@ int lumx_glossy(lumx_world_t w, const uint8_t* matnorm, const float* depth, int 
         HIP_TRY(hipEventRecord(w->ev[2][1], s));
         w->ev_valid[2] = true;
     }
+    return LUMX_OK;
+}
+
+int lumx_diffuse(lumx_world_t w, const uint8_t* matnorm, const float* depth, int width, int height,
+                 const lumx_camera* cam, const lumx_lightmap* lm, float* out, void* stream) {
+    if (!w || !matnorm || !depth || !cam || !out) return fail(LUMX_EINVAL, "null argument");
+    REQUIRE_RESIDENT(w);
+    if (width <= 0 || height <= 0 || width > 16384 || height > 16384)
+        return fail(LUMX_EINVAL, "frame %dx%d invalid", width, height);
+    if (lm && lm->depth_dev && (lm->width <= 0 || lm->height <= 0 || lm->width > 16384 || lm->height > 16384))
+        return fail(LUMX_EINVAL, "lightmap %dx%d invalid", lm->width, lm->height);
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    DiffuseArgs a{};
+    a.world = view_of(w);
+    a.field = w->field[w->cur];
+    a.matnorm = (const uchar4*)matnorm;
+    a.depth = depth;
+    a.out = (float4*)out;
+    a.width = width;
+    a.height = height;
+    for (int k = 0; k < 3; ++k) {
+        a.campos[k] = cam->campos[k];
+        a.camdir[k] = cam->camdir[k];
+        a.horiz[k] = cam->horiz_scaled[k];
+        a.verti[k] = cam->verti_scaled[k];
+        a.light[k] = cam->light_dir[k];
+    }
+    a.frame_w = cam->frame_size[0];
+    a.frame_h = cam->frame_size[1];
+    if (lm && lm->depth_dev) {
+        a.lightmap = lm->depth_dev;
+        a.lm_w = lm->width;
+        a.lm_h = lm->height;
+        for (int k = 0; k < 16; ++k) a.lm_proj[k] = lm->proj[k];
+    }
+    HIP_TRY(launch_diffuse(a, w->desc.radiance_format, s));
     return LUMX_OK;
 }
 

@@ -65,6 +65,9 @@ struct lumx_world {
     uint32_t* empty_bits = nullptr;     // device: per block, 1 = id 0
     uint32_t* pal_bits = nullptr;       // device: per slot, 128 words of occupancy bits
     unsigned int* batch_counter = nullptr;  // device: tiled kernel work counter (2 words)
+    unsigned long long* sel_bits = nullptr; // device probe selection: flags, one word per 64 probes
+    int* sel_tiles = nullptr;               // device probe selection: per-tile counts / offsets
+    int* sel_claim = nullptr;               // device probe selection: per probe special claim
     int16_t* stage_blocks = nullptr;    // pinned host staging of the blockified world
     hipEvent_t stage_done = nullptr;    // recorded after the staging uploads of the last map
     bool stage_pending = false;

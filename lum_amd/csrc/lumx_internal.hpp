@@ -18,6 +18,7 @@ struct RadianceArgs {
     int rays;
     int log2_rays;          // log2(rays) when rays is a power of two, else -1
     int n_probes;           // dispatched probes (= n_requests / size)
+    const int* n_requests_dev;  // device request count (nullptr = n_probes is exact)
     int time_seed;
     int size;
     int shift;
@@ -79,6 +80,51 @@ struct GlossyArgs {
     int max_steps;
 };
 
+struct DiffuseArgs {
+    WorldView world;
+    const void* field;
+    const uchar4* matnorm;
+    const float* depth;
+    float4* out;
+    int width, height;
+    float campos[3], camdir[3], horiz[3], verti[3], light[3];
+    float frame_w, frame_h;
+    const uint16_t* lightmap;  // D16 shadow map (nullptr = no sun term)
+    int lm_w, lm_h;
+    float lm_proj[16];         // column-major mat4
+};
+
+// Device probe selection (probes.hip).
+struct SelectArgs {
+    const int16_t* blocks;      // current world, x-fastest
+    int wx, wy, wz, n;          // n = wx * wy * wz
+    int n_tiles;                // select_tiles(n)
+    unsigned long long* bits;   // one ballot word per 64 probes
+    int* tile_counts;           // per tile: count, then offset
+    const int4* special;        // special requests (x, y, z, w)
+    int n_special;
+    int* claim;                 // per probe: first special index, idle value otherwise
+    int4* out;                  // n + n_special entries
+    int* count;                 // entries written
+};
+
+// Field shift (probes.hip): texel p + shift lands on p inside the destination box.
+struct ShiftArgs {
+    const void* in;
+    void* out;
+    int wx, wy, wz;
+    long long n;
+    int shift[3];
+    int dst[3];
+    int extent[3];
+};
+
+size_t select_tiles(int n_probes);
+size_t select_bits_bytes(int n_probes);
+int select_claim_idle();
+hipError_t launch_select(const SelectArgs& a, hipStream_t stream);
+hipError_t launch_shift(const ShiftArgs& a, int texel_bytes, hipStream_t stream);
+
 hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, hipStream_t stream);
 hipError_t launch_radiance_tiled(RadianceArgs a, int store, bool traced, const RadianceTune& t, int n_cu,
                                  hipStream_t stream);
@@ -89,6 +135,7 @@ hipError_t launch_palette_bits(const uint8_t* palette, int first_slot, int n_slo
 hipError_t launch_map(const MapArgs& a, hipStream_t stream);
 hipError_t launch_exec_copies(const CopyArgs& a, hipStream_t stream);
 hipError_t launch_glossy(const GlossyArgs& a, int store, bool traced, hipStream_t stream);
+hipError_t launch_diffuse(const DiffuseArgs& a, int store, hipStream_t stream);
 hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
                                  int total_threads, int block, int store, bool served, const int* frame_dev,
                                  hipStream_t stream);

@@ -6,6 +6,7 @@
 //   GetBlock / voxel_in_palette / GetVoxel  shaders/radiance.comp:97-143
 //   GetMat                                  shaders/radiance.comp:145-157
 //   sample_radiance                         shaders/radiance.comp:161-166
+//   (sample_texel also serves diffuse.frag:92-97)
 //   CastRay_fast                            shaders/radiance.comp:183-291
 // Arithmetic rules: no FMA contraction (the pragma below and
 // -ffp-contract=off), IEEE division and sqrt, fminf/fmaxf, float->int
@@ -24,21 +25,25 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Every helper here compiles for the host too, so the kernels' arithmetic can
+// be exercised on the CPU from the same source.
+#define LUMX_HD __host__ __device__ __forceinline__
+
 namespace lumx {
 
 struct f3 {
     float x, y, z;
 };
 
-__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
-__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
-__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ f3 normalize3(f3 v) {
+LUMX_HD f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+LUMX_HD f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+LUMX_HD f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+LUMX_HD f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+LUMX_HD f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+LUMX_HD f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+LUMX_HD f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+LUMX_HD float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+LUMX_HD f3 normalize3(f3 v) {
     float len = sqrtf(dot3(v, v));
     return {v.x / len, v.y / len, v.z / len};
 }
@@ -55,9 +60,9 @@ struct WorldView {
 // Radiance texel storage formats.
 enum { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2 };
 
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
+LUMX_HD float bf16_to_f32(uint16_t h) { return __builtin_bit_cast(float, ((uint32_t)h) << 16); }
+LUMX_HD uint16_t f32_to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
@@ -66,22 +71,22 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 template <int STORE> struct Texel;
 template <> struct Texel<STORE_F32> {
     typedef float4 type;
-    __device__ __forceinline__ static f3 load(const void* base, unsigned i) {
+    LUMX_HD static f3 load(const void* base, unsigned i) {
         float4 v = ((const float4*)base)[i];
         return {v.x, v.y, v.z};
     }
-    __device__ __forceinline__ static void store(void* base, unsigned i, f3 v) {
+    LUMX_HD static void store(void* base, unsigned i, f3 v) {
         ((float4*)base)[i] = make_float4(v.x, v.y, v.z, 1.0f);
     }
 };
 template <> struct Texel<STORE_F16> {
-    __device__ __forceinline__ static f3 load(const void* base, unsigned i) {
+    LUMX_HD static f3 load(const void* base, unsigned i) {
         uint2 raw = ((const uint2*)base)[i];
         _Float16 h[4];
         __builtin_memcpy(h, &raw, 8);
         return {(float)h[0], (float)h[1], (float)h[2]};
     }
-    __device__ __forceinline__ static void store(void* base, unsigned i, f3 v) {
+    LUMX_HD static void store(void* base, unsigned i, f3 v) {
         _Float16 h[4] = {(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)1.0f};
         uint2 raw;
         __builtin_memcpy(&raw, h, 8);
@@ -89,12 +94,12 @@ template <> struct Texel<STORE_F16> {
     }
 };
 template <> struct Texel<STORE_BF16> {
-    __device__ __forceinline__ static f3 load(const void* base, unsigned i) {
+    LUMX_HD static f3 load(const void* base, unsigned i) {
         uint2 raw = ((const uint2*)base)[i];
         return {bf16_to_f32((uint16_t)(raw.x & 0xffffu)), bf16_to_f32((uint16_t)(raw.x >> 16)),
                 bf16_to_f32((uint16_t)(raw.y & 0xffffu))};
     }
-    __device__ __forceinline__ static void store(void* base, unsigned i, f3 v) {
+    LUMX_HD static void store(void* base, unsigned i, f3 v) {
         uint2 raw;
         raw.x = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
         raw.y = (uint32_t)f32_to_bf16(v.z) | ((uint32_t)0x3f80u << 16);
@@ -102,24 +107,47 @@ template <> struct Texel<STORE_BF16> {
     }
 };
 
-// GetBlock + palette fetch for an integer voxel position.
-__device__ __forceinline__ int get_voxel(const WorldView& w, int ix, int iy, int iz, int* blk_out) {
-    int bx = ix / 16, by = iy / 16, bz = iz / 16;
-    int b = 0;
-    if ((unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz)
-        b = w.blocks[(unsigned)(bx + w.wx * (by + w.wy * bz))];
-    *blk_out = b;
+// a * b for operands below 2^24 (a full-rate 24-bit multiply on the GPU).
+LUMX_HD unsigned mul24(unsigned a, unsigned b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
+}
+
+// Block id at in-bounds block coordinates.  Every operand is below 2^24
+// (axes <= 4096, wy * wz <= 2^24) and the byte offset is formed in 32 bits
+// (worlds hold at most 2^28 blocks), so the load takes a scalar base plus a
+// 32-bit vector offset.
+LUMX_HD int block_id(const WorldView& w, int bx, int by, int bz) {
+    const unsigned lin = (unsigned)bx + mul24((unsigned)w.wx, (unsigned)by + mul24((unsigned)w.wy, (unsigned)bz));
+    return *(const int16_t*)((const char*)w.blocks + (lin << 1));
+}
+
+// voxel_in_palette: voxel (ix, iy, iz) mod 16 of block b.
+LUMX_HD int voxel_in_block(const WorldView& w, int b, int ix, int iy, int iz) {
     if (b == 0 && w.block0_empty) return 0;
     if ((unsigned)b >= (unsigned)w.palette_slots) return 0;
     uint32_t off = ((uint32_t)b << 12) | (uint32_t)((ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8));
     return w.palette[off];
 }
 
+// GetBlock + palette fetch for an integer voxel position.
+LUMX_HD int get_voxel(const WorldView& w, int ix, int iy, int iz, int* blk_out) {
+    int bx = ix / 16, by = iy / 16, bz = iz / 16;
+    int b = 0;
+    if ((unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz)
+        b = block_id(w, bx, by, bz);
+    *blk_out = b;
+    return voxel_in_block(w, b, ix, iy, iz);
+}
+
 struct MatF {
     f3 color;
     float emit, rough;
 };
-__device__ __forceinline__ MatF get_mat(const WorldView& w, int v) {
+LUMX_HD MatF get_mat(const WorldView& w, int v) {
     const float* m = w.mats + v * 6;
     MatF r;
     r.color = {m[0], m[1], m[2]};
@@ -129,7 +157,7 @@ __device__ __forceinline__ MatF get_mat(const WorldView& w, int v) {
 }
 
 // i mod n in [0, n); one conditional add or subtract covers i in [-n, 2n).
-__device__ __forceinline__ int wrap_i(int i, int n) {
+LUMX_HD int wrap_i(int i, int n) {
     int r = i < 0 ? i + n : (i >= n ? i - n : i);
     if ((unsigned)r >= (unsigned)n) {
         r = i % n;
@@ -138,10 +166,10 @@ __device__ __forceinline__ int wrap_i(int i, int n) {
     return r;
 }
 
-// sample_radiance: trilinear at texel coordinate (x + n*16)/16, REPEAT wrap.
+// Trilinear LINEAR filter with REPEAT wrap at texel-space coordinate bp
+// (texel i's centre at bp = i, i.e. normalized coordinate (bp + 0.5) / size).
 template <int STORE>
-__device__ __forceinline__ f3 sample_radiance(const void* field, int wx, int wy, int wz, f3 x, f3 n) {
-    f3 bp = (x + n * 16.0f) / 16.0f;
+LUMX_HD f3 sample_texel(const void* field, int wx, int wy, int wz, f3 bp) {
     float fx0 = floorf(bp.x), fy0 = floorf(bp.y), fz0 = floorf(bp.z);
     float tx = bp.x - fx0, ty = bp.y - fy0, tz = bp.z - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
@@ -170,6 +198,12 @@ __device__ __forceinline__ f3 sample_radiance(const void* field, int wx, int wy,
     return c0 * uz + c1 * tz;
 }
 
+// sample_radiance: trilinear at texel coordinate (x + n*16)/16, REPEAT wrap.
+template <int STORE>
+LUMX_HD f3 sample_radiance(const void* field, int wx, int wy, int wz, f3 x, f3 n) {
+    return sample_texel<STORE>(field, wx, wy, wz, (x + n * 16.0f) / 16.0f);
+}
+
 struct RayHit {
     int hit;
     int voxel;
@@ -182,7 +216,7 @@ struct RayHit {
 
 // CastRay_fast: coarse 0.5-voxel march with empty-block skipping, then up to
 // five voxel-DDA refinement steps from origin + dir*(fraction - 1.5).
-__device__ __forceinline__ RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_steps) {
+LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_steps) {
     RayHit r;
     r.hit = 0;
     r.voxel = 0;
@@ -193,20 +227,23 @@ __device__ __forceinline__ RayHit cast_ray(const WorldView& w, f3 o, f3 d, float
     r.key = 0;
     const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     const f3 corner = {d.x > 0.0f ? 16.0f : 0.0f, d.y > 0.0f ? 16.0f : 0.0f, d.z > 0.0f ? 16.0f : 0.0f};
-    const float ext_x = (float)(w.wx * 16), ext_y = (float)(w.wy * 16), ext_z = (float)(w.wz * 16);
     float fraction = 0.0f;
     int it = 0;
     int cur = 0;
     while (true) {
         fraction += 0.5f;
         f3 pos = o + d * fraction;
-        int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
-        int blk;
-        cur = get_voxel(w, ix, iy, iz, &blk);
+        const int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
+        // Block coordinate ivec3(p) / 16 (truncating) as trunc(p / 16): the
+        // same integer, p / 16 being exact, without the signed division.
+        const int bx = (int)(pos.x * 0.0625f), by = (int)(pos.y * 0.0625f), bz = (int)(pos.z * 0.0625f);
+        const bool inb =
+            (unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz;
+        const int blk = inb ? block_id(w, bx, by, bz) : 0;
+        cur = voxel_in_block(w, blk, ix, iy, iz);
         r.coarse++;
         if (blk == 0) {
-            f3 box = {(float)((ix / 16) * 16) + corner.x, (float)((iy / 16) * 16) + corner.y,
-                      (float)((iz / 16) * 16) + corner.z};
+            f3 box = {(float)(bx * 16) + corner.x, (float)(by * 16) + corner.y, (float)(bz * 16) + corner.z};
             f3 tmp = (-pos) * inv;
             f3 bc = box * inv + tmp;
             float f = fminf(bc.x, fminf(bc.y, bc.z));
@@ -217,8 +254,9 @@ __device__ __forceinline__ RayHit cast_ray(const WorldView& w, f3 o, f3 d, float
             r.hit = 1;
             break;
         }
-        if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || pos.x >= ext_x || pos.y >= ext_y || pos.z >= ext_z)
-            break;
+        // p outside [0, 16 W): a negative coordinate, or (p >= 0) a block
+        // coordinate past the far edge, since trunc(p / 16) < W <=> p < 16 W.
+        if (!inb || fminf(pos.x, fminf(pos.y, pos.z)) < 0.0f) break;
         if (fraction >= max_dist) break;
         if (max_steps > 0 && ++it >= max_steps) break;
     }
@@ -258,7 +296,7 @@ __device__ __forceinline__ RayHit cast_ray(const WorldView& w, f3 o, f3 d, float
 }
 
 // globalLightDir of radiance.comp:42, normalize(vec3(0.5, 0.5, -0.9)) in FP32.
-__device__ __forceinline__ f3 radiance_light_dir() { return normalize3(f3{0.5f, 0.5f, -0.9f}); }
+LUMX_HD f3 radiance_light_dir() { return normalize3(f3{0.5f, 0.5f, -0.9f}); }
 
 }  // namespace lumx
 
@@ -279,27 +317,27 @@ struct OccView {
     int palette_slots;
     int block0_empty;
 
-    __device__ __forceinline__ int block_at(int ix, int iy, int iz) const {
+    LUMX_HD int block_at(int ix, int iy, int iz) const {
         int bx = ix / 16, by = iy / 16, bz = iz / 16;
         if ((unsigned)bx >= (unsigned)wx || (unsigned)by >= (unsigned)wy || (unsigned)bz >= (unsigned)wz) return 0;
         unsigned lin = (unsigned)(bx + wx * (by + wy * bz));
         if ((eb[lin >> 5] >> (lin & 31)) & 1u) return 0;
         return ids[lin];
     }
-    __device__ __forceinline__ bool occupied(int b, int ix, int iy, int iz) const {
+    LUMX_HD bool occupied(int b, int ix, int iy, int iz) const {
         if (b == 0 && block0_empty) return false;
         if ((unsigned)b >= (unsigned)palette_slots) return false;
         int idx = (ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8);
         return (pb[((uint32_t)b << 7) | (uint32_t)(idx >> 5)] >> (idx & 31)) & 1u;
     }
-    __device__ __forceinline__ int voxel_value(int b, int ix, int iy, int iz) const {
+    LUMX_HD int voxel_value(int b, int ix, int iy, int iz) const {
         int idx = (ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8);
         return palette[((uint32_t)b << 12) | (uint32_t)idx];
     }
 };
 
 // CastRay_fast over an OccView; same arithmetic and results as cast_ray().
-__device__ __forceinline__ RayHit cast_ray_occ(const OccView& w, f3 o, f3 d, float max_dist, int max_steps) {
+LUMX_HD RayHit cast_ray_occ(const OccView& w, f3 o, f3 d, float max_dist, int max_steps) {
     RayHit r;
     r.hit = 0;
     r.voxel = 0;

@@ -30,6 +30,13 @@ __device__ __forceinline__ void hash33(uint32_t x, uint32_t y, uint32_t z, f3* o
     *out = {(float)x * inv, (float)y * inv, (float)z * inv};
 }
 
+// Probes of this launch: n_probes, or fewer when the request count lives on
+// the device (the reference's wg_count = n_requests / size).
+__device__ __forceinline__ int dispatched_probes(const RadianceArgs& a) {
+    if (!a.n_requests_dev) return a.n_probes;
+    return min(a.n_probes, *a.n_requests_dev / a.size);
+}
+
 // One ray per thread; tid is the thread's index within this world's launch
 // range, which starts on a workgroup boundary.
 template <int STORE, bool TRACED, bool OCC>
@@ -39,7 +46,7 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
     // With R >= 64 a wave holds rays of one probe: keep the probe on the scalar unit.
     if (R >= 64) wg = __builtin_amdgcn_readfirstlane(wg);
     const int lane = tid - wg * R;
-    const bool active = wg < a.n_probes;
+    const bool active = wg < dispatched_probes(a);
 
     const WorldView& w = a.world;
     int px = 0, py = 0, pz = 0;
@@ -296,7 +303,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
             f3 light = {0.0f, 0.0f, 0.0f};
             int px = 0, py = 0, pz = 0;
             bool valid = false;
-            if (wg < a.n_probes) {
+            if (wg < dispatched_probes(a)) {
                 if (a.requests) {
                     int4 rq = a.requests[wg * a.size + a.shift];
                     px = rq.x;
@@ -371,7 +378,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
         }
         for (int p = tid; p < P; p += T) {
             const int wg = batch * P + p;
-            if (wg >= a.n_probes) continue;
+            if (wg >= dispatched_probes(a)) continue;
             int px, py, pz;
             if (a.requests) {
                 int4 rq = a.requests[wg * a.size + a.shift];

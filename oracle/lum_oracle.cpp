@@ -21,6 +21,8 @@
 //   src/renderer/src/internal_render.cpp:977-1097,2182-2186 blockify  -> blockify()
 //   src/renderer/src/internal_render.cpp:1142-1168 probe selection    -> select_probes()
 //   src/renderer/src/internal_render.cpp:1375-1476 exec_copies        -> exec_copies()
+//   src/renderer/src/internal_render.cpp:1218-1335 shift_radiance     -> shift_radiance()
+//   shaders/diffuse.frag:92-97,105-137,161-245 diffuse main            -> diffuse()
 //
 // Arithmetic conventions (shared, by definition, with the HIP kernels):
 //   * every float expression is evaluated unfused, left to right, in the order
@@ -161,8 +163,9 @@ inline int wrap(int i, int n) { int r = i % n; return r < 0 ? r + n : r; }
 // sample_radiance (radiance.comp:161-166): trilinear at texel coordinate
 // bp = (x + n*16)/16 with REPEAT wrap (normalized LINEAR sampler, the
 // texel-centre mapping (bp+0.5)/W * W - 0.5 == bp is taken exactly).
-template <class T> V3<T> sample_radiance(const Field<T>& f, V3<T> x, V3<T> n) {
-    V3<T> bp = (x + n * (T)16) / (T)16;
+// Trilinear LINEAR filter with REPEAT wrap at texel-space coordinate bp
+// (texel i's centre at bp = i), i.e. at normalized (bp + 0.5) / size.
+template <class T> V3<T> sample_texel(const Field<T>& f, V3<T> bp) {
     T fx0 = std::floor(bp.x), fy0 = std::floor(bp.y), fz0 = std::floor(bp.z);
     T tx = bp.x - fx0, ty = bp.y - fy0, tz = bp.z - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
@@ -184,6 +187,10 @@ template <class T> V3<T> sample_radiance(const Field<T>& f, V3<T> x, V3<T> n) {
     V3<T> c0 = lerp(c00, c10, ty);
     V3<T> c1 = lerp(c01, c11, ty);
     return lerp(c0, c1, tz);
+}
+
+template <class T> V3<T> sample_radiance(const Field<T>& f, V3<T> x, V3<T> n) {
+    return sample_texel<T>(f, (x + n * (T)16) / (T)16);
 }
 
 template <class T> struct RayResult {
@@ -667,6 +674,117 @@ void orc_glossy_f32(const orc_world* w, const float* field, const uint8_t* matno
         });
     }
     for (auto& th : pool) th.join();
+}
+
+// --------------------------------------------------------------------------
+// diffuse.frag:228-245 (main) over a W x H G-buffer, with
+// sample_radiance(position) (:92-97: textureLod at (position/16)/world_size,
+// LINEAR + REPEAT, i.e. texel-space coordinate position/16 - 0.5),
+// get_origin_from_depth (:131-137), load_norm/load_mat/load_depth (:105-118)
+// and sample_lightmap (:154-218: five NEAREST shadow taps 1/1024 apart, depth
+// compare LESS against the D16 lightmap, MIRRORED_REPEAT).  lightmap may be
+// null: the sun term is then 0.  The reference value of the compare is
+// clamped to [0, 1] (a UNORM depth format); the stored depth is v / 65535.
+// ubo: as orc_glossy_f32; lm_proj: column-major mat4 (ubo.lightmap_proj).
+// out: float4 per pixel = (final / 8, 1).
+// --------------------------------------------------------------------------
+static int mirror_repeat(int i, int size) {
+    int m = i % (2 * size);
+    if (m < 0) m += 2 * size;
+    int a = m - size;
+    int mir = a >= 0 ? a : -(1 + a);
+    return (size - 1) - mir;
+}
+
+void orc_diffuse_f32(const orc_world* w, const float* field, const uint8_t* matnorm, const float* depth,
+                     int width, int height, const float* ubo, const uint16_t* lightmap, int lm_w, int lm_h,
+                     const float* lm_proj, float* out) {
+    World ww = to_world(w);
+    Field<float> f{w->wx, w->wy, w->wz, field};
+    V3<float> campos = {ubo[0], ubo[1], ubo[2]};
+    V3<float> camdir = {ubo[3], ubo[4], ubo[5]};
+    V3<float> horiz = {ubo[6], ubo[7], ubo[8]};
+    V3<float> verti = {ubo[9], ubo[10], ubo[11]};
+    V3<float> gl = {ubo[12], ubo[13], ubo[14]};
+    float fw = ubo[15], fh = ubo[16];
+    for (int py = 0; py < height; ++py)
+        for (int px = 0; px < width; ++px) {
+            size_t pi = (size_t)py * width + px;
+            const uint8_t* mn = matnorm + 4 * pi;
+            Mat<float> mat = get_mat<float>(ww, mn[0]);
+            float cx = ((float)px + 0.5f) / fw * 2.0f - 1.0f;
+            float cy = ((float)py + 0.5f) / fh * 2.0f - 1.0f;
+            float dep = depth[pi] * 1000.0f;
+            V3<float> origin = campos + horiz * cx + verti * cy + camdir * dep;
+            V3<float> normal = {((float)mn[1] / 255.0f) * 2.0f - 1.0f, ((float)mn[2] / 255.0f) * 2.0f - 1.0f,
+                                ((float)mn[3] / 255.0f) * 2.0f - 1.0f};
+            V3<float> pos = origin + normal * 6.0f;
+            V3<float> bp = pos / 16.0f;
+            V3<float> incoming = sample_texel<float>(f, V3<float>{bp.x - 0.5f, bp.y - 0.5f, bp.z - 0.5f});
+            float sun = 0.0f;
+            if (lightmap) {
+                V3<float> biased = origin;
+                if (dot(normal, gl) > 0.0f) biased = biased - normal * 0.9f;
+                else biased = biased + normal * 0.9f;
+                float c[3];
+                for (int r = 0; r < 3; ++r)
+                    c[r] = lm_proj[0 * 4 + r] * biased.x + lm_proj[1 * 4 + r] * biased.y +
+                           lm_proj[2 * 4 + r] * biased.z + lm_proj[3 * 4 + r];
+                float world_depth = 1.0f + c[2];
+                float u = (c[0] + 1.0f) / 2.0f, v = (c[1] + 1.0f) / 2.0f;
+                float dref = std::min(std::max(world_depth, 0.0f), 1.0f);
+                const int taps[5][2] = {{-1, 0}, {0, 0}, {1, 0}, {0, -1}, {0, 1}};
+                float total = 0.0f;
+                for (int k = 0; k < 5; ++k) {
+                    float su = u + (float)taps[k][0] * (1.0f / 1024.0f);
+                    float sv = v + (float)taps[k][1] * (1.0f / 1024.0f);
+                    int i = (int)std::floor(su * (float)lm_w), j = (int)std::floor(sv * (float)lm_h);
+                    i = mirror_repeat(i, lm_w);
+                    j = mirror_repeat(j, lm_h);
+                    float dtex = (float)lightmap[(size_t)j * lm_w + i] / 65535.0f;
+                    total = total + (dref < dtex ? 1.0f : 0.0f);
+                }
+                sun = (total / 5.0f) * 0.15f;
+            }
+            V3<float> fin = {(2.0f * incoming.x + mat.emit + sun) * mat.color.x,
+                             (2.0f * incoming.y + mat.emit + sun) * mat.color.y,
+                             (2.0f * incoming.z + mat.emit + sun) * mat.color.z};
+            float* o = out + 4 * pi;
+            o[0] = fin.x / 8.0f; o[1] = fin.y / 8.0f; o[2] = fin.z / 8.0f; o[3] = 1.0f;
+        }
+}
+
+// shift_radiance (internal_render.cpp:1218-1335): per axis, source offset =
+// max(shift, 0), destination offset = max(-shift, 0), extent = size - |shift|;
+// the intersection is copied to a temporary image at the origin and from there
+// to the destination offset of the field.  texel_floats = floats per texel.
+// The reference tests only |shift.x| >= size.x (three times); this
+// restatement leaves the field unchanged when any axis has |shift| >= size.
+void orc_shift_radiance(int wx, int wy, int wz, int texel_floats, float* field, int sx, int sy, int sz) {
+    const int size[3] = {wx, wy, wz}, shift[3] = {sx, sy, sz};
+    int src[3], dst[3], ext[3];
+    for (int k = 0; k < 3; ++k) {
+        if (std::abs(shift[k]) >= size[k]) return;
+        src[k] = shift[k] >= 0 ? shift[k] : 0;
+        dst[k] = shift[k] >= 0 ? 0 : -shift[k];
+        ext[k] = size[k] - std::abs(shift[k]);
+    }
+    const int c = texel_floats;
+    std::vector<float> temp((size_t)ext[0] * ext[1] * ext[2] * c);
+    for (int z = 0; z < ext[2]; ++z)
+        for (int y = 0; y < ext[1]; ++y)
+            for (int x = 0; x < ext[0]; ++x) {
+                const size_t t = ((size_t)x + (size_t)ext[0] * (y + (size_t)ext[1] * z)) * c;
+                const size_t f = ((size_t)(x + src[0]) + (size_t)wx * ((y + src[1]) + (size_t)wy * (z + src[2]))) * c;
+                for (int i = 0; i < c; ++i) temp[t + i] = field[f + i];
+            }
+    for (int z = 0; z < ext[2]; ++z)
+        for (int y = 0; y < ext[1]; ++y)
+            for (int x = 0; x < ext[0]; ++x) {
+                const size_t t = ((size_t)x + (size_t)ext[0] * (y + (size_t)ext[1] * z)) * c;
+                const size_t f = ((size_t)(x + dst[0]) + (size_t)wx * ((y + dst[1]) + (size_t)wy * (z + dst[2]))) * c;
+                for (int i = 0; i < c; ++i) field[f + i] = temp[t + i];
+            }
 }
 
 }  // extern "C"

@@ -173,3 +173,32 @@ def glossy(world, field, matnorm, depth, ubo17, max_dist=128.0, max_steps=0, thr
     load().orc_glossy_f32(C.byref(world.c), _p(f), _p(mn), _p(dp), w, h, _p(u), C.c_float(max_dist),
                           C.c_int(max_steps), _p(out), _p(keys), threads)
     return out, keys
+
+
+def shift_radiance(field, shift):
+    """field: float32 (wz, wy, wx, c); returns the shifted copy."""
+    f = np.ascontiguousarray(field, np.float32).copy()
+    wz, wy, wx, c = f.shape
+    load().orc_shift_radiance(wx, wy, wz, c, _p(f), int(shift[0]), int(shift[1]), int(shift[2]))
+    return f
+
+
+def diffuse(world, field, matnorm, depth, ubo17, lightmap=None, lm_proj=None):
+    """diffuse.frag main; lightmap: uint16 (h, w) or None; lm_proj: 16
+    floats column-major."""
+    f = np.ascontiguousarray(field, np.float32)
+    mn = np.ascontiguousarray(matnorm, np.uint8)
+    dp = np.ascontiguousarray(depth, np.float32)
+    h, w = dp.shape
+    u = np.ascontiguousarray(ubo17, np.float32)
+    out = np.zeros((h, w, 4), np.float32)
+    if lightmap is None:
+        lm, lw, lh = None, 0, 0
+        pr = np.zeros(16, np.float32)
+    else:
+        lm = np.ascontiguousarray(lightmap, np.uint16)
+        lh, lw = lm.shape
+        pr = np.ascontiguousarray(lm_proj, np.float32).reshape(16)
+    load().orc_diffuse_f32(C.byref(world.c), _p(f), _p(mn), _p(dp), w, h, _p(u),
+                           None if lm is None else _p(lm), lw, lh, _p(pr), _p(out))
+    return out

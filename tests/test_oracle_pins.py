@@ -8,7 +8,8 @@ Reference semantics pinned here (paths relative to the reference repository):
   :316-354 trace_ray + store_probe; shaders/map.comp:33-68;
   shaders/glossy.frag:312-360,444-493;
   src/renderer/src/internal_render.cpp:1029-1076 (blockify), 1142-1168 (probe
-  selection), 1485-1518 (map_mesh placement), 923-934 (camera).
+  selection), 1218-1335 (shift_radiance), 1485-1518 (map_mesh placement),
+  923-934 (camera).
 """
 import math
 
@@ -473,6 +474,37 @@ def test_select_probes_neighbourhood_and_specials():
     assert keys == sorted(keys)
 
 
+# ---------------------------------------------------------------- shift_radiance
+@pytest.mark.parametrize("shift", [(1, 0, 0), (-1, 0, 0), (0, 2, -1), (-3, -2, 1), (2, 3, 4), (0, 0, 0)])
+def test_shift_radiance_moves_texel_p_plus_shift_to_p(shift):
+    """Pointwise statement of the two image copies: new[p] = old[p + shift]
+    where both p and p + shift lie in the world, else new[p] = old[p]."""
+    wx, wy, wz = 7, 6, 5
+    rng = np.random.default_rng(3)
+    old = rng.random((wz, wy, wx, 4)).astype(np.float32)
+    got = orc.shift_radiance(old, shift)
+    for z in range(wz):
+        for y in range(wy):
+            for x in range(wx):
+                sx, sy, sz = x + shift[0], y + shift[1], z + shift[2]
+                if 0 <= sx < wx and 0 <= sy < wy and 0 <= sz < wz:
+                    assert np.array_equal(got[z, y, x], old[sz, sy, sx])
+                else:
+                    assert np.array_equal(got[z, y, x], old[z, y, x])
+
+
+def test_shift_radiance_whole_size_is_identity_and_inverse_restores_core():
+    rng = np.random.default_rng(4)
+    old = rng.random((5, 6, 7, 4)).astype(np.float32)
+    for s in [(7, 0, 0), (0, -6, 0), (0, 0, 5), (-9, 1, 1)]:
+        assert np.array_equal(orc.shift_radiance(old, s), old)
+    # shifting by s then -s restores every texel that went out and back
+    # inside the world: x in [2, 7), z in [0, 4)
+    back = orc.shift_radiance(orc.shift_radiance(old, (2, 0, -1)), (-2, 0, 1))
+    assert np.array_equal(back[:4, :, 2:], old[:4, :, 2:])
+    assert not np.array_equal(back, old)
+
+
 # ---------------------------------------------------------------- glossy
 def test_glossy_sky_closed_form_and_mask():
     from lum_amd.api import camera_default
@@ -505,6 +537,99 @@ def test_glossy_sky_closed_form_and_mask():
     assert np.allclose(out[m][:, :3], expect, rtol=1e-5, atol=1e-7)
     assert np.allclose(out[m][:, 3], 1 - mats[1, 5])
     assert np.all(out[1, 3] == 0) and np.all(keys == 0)
+
+
+# ---------------------------------------------------------------- diffuse
+def diffuse_ubo(campos, light=(0.5, 0.5, -0.9), frame=(4, 2)):
+    """Camera with zero direction and image-plane vectors: origin = campos."""
+    return np.array(list(campos) + [0.0] * 9 + list(light) + list(frame), np.float32)
+
+
+def diffuse_gbuffer(material):
+    mn = np.zeros((2, 4, 4), np.uint8)
+    mn[..., 0] = material
+    mn[..., 1:] = 255      # normal decodes to (1, 1, 1), not renormalised
+    return mn, np.zeros((2, 4), np.float32)
+
+
+def test_diffuse_samples_the_probe_at_origin_plus_six_normals():
+    """textureLod at (position/16)/world_size: a position at a probe centre
+    (16 i + 8) reads texel i exactly; one block past the edge wraps (REPEAT);
+    half a block between centres is the mean of the two texels."""
+    wx, wy, wz = 5, 4, 3
+    rng = np.random.default_rng(5)
+    f = field_of(wx, wy, wz, rng)
+    mats = np.zeros((256, 6), np.float32)
+    mats[3] = [1, 1, 1, 1, 0, 0]
+    w = world(np.zeros((wz, wy, wx), np.int16), np.zeros((1, 16, 16, 16), np.uint8), mats)
+    mn, dp = diffuse_gbuffer(3)
+    for (i, j, k), texel in [((0, 0, 0), (0, 0, 0)), ((4, 3, 2), (4, 3, 2)), ((2, 1, 1), (2, 1, 1)),
+                             ((wx, 0, 0), (0, 0, 0)), ((-1, 0, 0), (wx - 1, 0, 0)), ((0, 0, wz), (0, 0, 0))]:
+        campos = (16 * i + 8 - 6, 16 * j + 8 - 6, 16 * k + 8 - 6)
+        out = orc.diffuse(w, f, mn, dp, diffuse_ubo(campos))
+        expect = 2 * f[texel[2], texel[1], texel[0], :3] / 8
+        assert np.array_equal(out[..., :3], np.broadcast_to(expect, out[..., :3].shape)), (i, j, k)
+        assert np.all(out[..., 3] == 1)
+    out = orc.diffuse(w, f, mn, dp, diffuse_ubo((16 * 1 + 8 + 8 - 6, 16 + 8 - 6, 16 + 8 - 6)))
+    mid = (f[1, 1, 1, :3] + f[1, 1, 2, :3])   # 2 * mean
+    assert np.allclose(out[0, 0, :3], mid / 8, atol=1e-6)
+
+
+def test_diffuse_material_emission_and_colour():
+    f = field_of(3, 3, 3, value=0.25)
+    mats = np.zeros((256, 6), np.float32)
+    mats[7] = [0.5, 0.25, 1.0, 1.0, 0.75, 0.3]
+    w = world(np.zeros((3, 3, 3), np.int16), np.zeros((1, 16, 16, 16), np.uint8), mats)
+    mn, dp = diffuse_gbuffer(7)
+    out = orc.diffuse(w, f, mn, dp, diffuse_ubo((20.0, 21.0, 22.0)))
+    expect = (2 * 0.25 + 0.75) * mats[7, :3] / 8
+    assert np.allclose(out[..., :3], expect, rtol=1e-6)
+
+
+@pytest.mark.parametrize("x0,lit_col", [(-1.125, 0), (-1.375, 1), (1.375, 6), (0.3, 5)])
+def test_diffuse_sun_taps_use_mirrored_repeat(x0, lit_col):
+    """Identity lightmap_proj: uv = (biased.xy + 1)/2 with biased = origin -
+    0.9 normal (normal . light > 0).  Texel column floor(u * 8) is mirrored
+    outside [0, 8): -1 -> 0, -2 -> 1, 9 -> 6 (clamping or repeating would
+    read other columns)."""
+    f = field_of(3, 3, 3, value=0.0)
+    mats = np.zeros((256, 6), np.float32)
+    mats[1] = [1, 1, 1, 1, 0, 0]
+    w = world(np.zeros((3, 3, 3), np.int16), np.zeros((1, 16, 16, 16), np.uint8), mats)
+    mn, dp = diffuse_gbuffer(1)
+    proj = np.eye(4, dtype=np.float32).reshape(16)
+    origin = (x0 + 0.9, 0.0 + 0.9, -0.5 + 0.9)          # biased = (x0, 0, -0.5): depth ref 0.5
+    lm = np.zeros((8, 8), np.uint16)
+    lm[:, lit_col] = 65535
+    out = orc.diffuse(w, f, mn, dp, diffuse_ubo(origin), lm, proj)
+    assert np.allclose(out[..., :3], 0.15 / 8, rtol=1e-6)
+    lm[:] = 65535
+    lm[:, lit_col] = 0
+    out = orc.diffuse(w, f, mn, dp, diffuse_ubo(origin), lm, proj)
+    assert np.all(out[..., :3] == 0)
+
+
+def test_diffuse_sun_depth_reference_is_clamped():
+    f = field_of(3, 3, 3, value=0.0)
+    mats = np.zeros((256, 6), np.float32)
+    mats[1] = [1, 1, 1, 1, 0, 0]
+    w = world(np.zeros((3, 3, 3), np.int16), np.zeros((1, 16, 16, 16), np.uint8), mats)
+    mn, dp = diffuse_gbuffer(1)
+    proj = np.eye(4, dtype=np.float32).reshape(16)
+    full, one = np.full((8, 8), 65535, np.uint16), np.full((8, 8), 1, np.uint16)
+    zero = np.zeros((8, 8), np.uint16)
+    # z = -3: reference depth -2 clamps to 0, lit against any stored depth > 0
+    o_low = (0.9, 0.9, -3.0 + 0.9)
+    assert np.allclose(orc.diffuse(w, f, mn, dp, diffuse_ubo(o_low), one, proj)[..., :3], 0.15 / 8)
+    assert np.all(orc.diffuse(w, f, mn, dp, diffuse_ubo(o_low), zero, proj)[..., :3] == 0)
+    # z = +0.5: reference depth 1.5 clamps to 1, never below a stored 1.0
+    o_high = (0.9, 0.9, 0.5 + 0.9)
+    assert np.all(orc.diffuse(w, f, mn, dp, diffuse_ubo(o_high), full, proj)[..., :3] == 0)
+    # normal . light <= 0 biases along +normal instead: z = -1.4 + 0.9 = -0.5
+    o_back = (-0.9, -0.9, -1.4)
+    out = orc.diffuse(w, f, mn, dp, diffuse_ubo(o_back, light=(-0.5, -0.5, -0.2)), full, proj)
+    assert np.allclose(out[..., :3], 0.15 / 8)
+    assert np.all(orc.diffuse(w, f, mn, dp, diffuse_ubo(o_back, light=(-0.5, -0.5, -0.2)), zero, proj)[..., :3] == 0)
 
 
 def test_camera_default_matches_lum_camera():
