@@ -117,6 +117,9 @@ void lumx_lum_request_radiance_update(LumRenderer instance, int x, int y, int z)
  * matnorm u8x4, depth = view depth / 1000, out float4); NULL detaches. */
 void lumx_lum_set_gbuffer(LumRenderer instance, const uint8_t* matnorm_dev, const float* depth_dev, int width,
                           int height, float* out_dev);
+/* Renderer::loadPalette (renderer.cpp:150-152): the Material table of a
+ * .vox file (upload it with lum_upload_material_palette_to_gpu). */
+void lumx_lum_load_palette_file(LumRenderer instance, const char* file);
 /* Renderer::setStoredRadianceShift / getStoredRadianceShift
  * (renderer.hpp:125-127, renderer.cpp:604-609): the shift applied to the
  * radiance field at every lum_submit_frame, before its radiance update. */

@@ -245,6 +245,17 @@ int lumx_glossy(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_d
  * normal + emission + sun) * colour / 8, 1).  lightmap may be NULL. */
 int lumx_diffuse(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_dev, int width, int height,
                  const lumx_camera* cam, const lumx_lightmap* lightmap, float* out_dev, void* stream);
+/* ---- MagicaVoxel .vox import ----
+ * The reference reads .vox through ogt_vox (common/ogt_vox.hpp,
+ * vox_read_scene) in load_mesh / load_block / extract_palette and builds its
+ * Material table in extract_palette_mem (src/renderer/src/load_stuff.cpp:
+ * 233-342).  Parses a .vox file image: size_out[3] gets the first distinct
+ * non-empty model's size; voxels (x-fastest colour indices, 0 = empty) is
+ * written when voxels_capacity >= size_x * size_y * size_z; materials (256 x
+ * {r, g, b, a, emission, roughness}) is written when non-NULL;
+ * n_models_out (optional) gets the number of distinct non-empty models. */
+int lumx_vox_parse(const void* data, size_t bytes, int* size_out, uint8_t* voxels, size_t voxels_capacity,
+                   float* materials, int* n_models_out);
 /* Lum's default camera (internal_types.hpp:54-66) for a frame size. */
 int lumx_camera_default(int frame_w, int frame_h, lumx_camera* out);
 

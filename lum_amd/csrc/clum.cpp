@@ -23,6 +23,7 @@
 
 #include "../../include/clum.h"
 #include "../../include/lumx.h"
+#include "host_internal.hpp"
 
 namespace {
 
@@ -51,6 +52,7 @@ struct Renderer {
     size_t special_cap = 0;          // entries special_dev holds
     std::vector<int32_t> special_up; // host source of special_dev until the frame ends
     int radiance_shift[3] = {0, 0, 0};  // Renderer::stored_radiance_shift
+    bool has_palette = false;           // a Material table was taken from a .vox file
     const uint8_t* gb_matnorm = nullptr;
     const float* gb_depth = nullptr;
     float* gb_out = nullptr;
@@ -160,6 +162,29 @@ bool model_visible(const int size[3], const LumMeshTransform& t, const double ca
                     return true;
             }
     return false;
+}
+
+bool read_file(const char* path, std::vector<uint8_t>* out) {
+    FILE* f = path ? std::fopen(path, "rb") : nullptr;
+    if (!f) return false;
+    uint8_t tmp[65536];
+    size_t n;
+    out->clear();
+    while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) out->insert(out->end(), tmp, tmp + n);
+    std::fclose(f);
+    return true;
+}
+
+// A .vox file's first model and Material table (load_stuff.cpp:233-342).
+bool parse_vox_file(const char* path, int size[3], std::vector<uint8_t>* voxels, float* mats) {
+    std::vector<uint8_t> buf;
+    if (!read_file(path, &buf)) {
+        lumx::fail(LUMX_EINVAL, "cannot read %s", path ? path : "(null)");
+        return false;
+    }
+    if (lumx_vox_parse(buf.data(), buf.size(), size, nullptr, 0, nullptr, nullptr) != LUMX_OK) return false;
+    voxels->assign((size_t)size[0] * size[1] * size[2], 0);
+    return lumx_vox_parse(buf.data(), buf.size(), size, voxels->data(), voxels->size(), mats, nullptr) == LUMX_OK;
 }
 
 void load_scene_file(Renderer* r, const char* path) {
@@ -288,8 +313,20 @@ void lum_load_block(LumRenderer instance, LumMeshBlock block, void* block_data) 
     std::memcpy(r->blocks[block].data(), block_data, 4096);
 }
 
-void lum_load_block_file(LumRenderer, LumMeshBlock, const char*) {
-    // .vox import is not part of this library (MagicaVoxel reader absent).
+void lum_load_block_file(LumRenderer instance, LumMeshBlock block, const char* file) {
+    // load_block (load_stuff.cpp:316-342): BlockVoxels[x][y][z] from the
+    // file's first model; a model larger than 16 per axis is cut to 16.
+    Renderer* r = get(instance);
+    if (!r || block < 0 || block >= (int)r->blocks.size()) return;
+    int size[3];
+    std::vector<uint8_t> v;
+    if (!parse_vox_file(file, size, &v, nullptr)) return;
+    std::array<uint8_t, 4096>& dst = r->blocks[block];
+    dst.fill(0);
+    for (int x = 0; x < std::min(size[0], 16); ++x)
+        for (int y = 0; y < std::min(size[1], 16); ++y)
+            for (int z = 0; z < std::min(size[2], 16); ++z)
+                dst[x * 256 + y * 16 + z] = v[x + (size_t)size[0] * (y + (size_t)size[1] * z)];
 }
 
 void lum_upload_block_palette_to_gpu(LumRenderer instance) {
@@ -305,7 +342,25 @@ void lum_upload_material_palette_to_gpu(LumRenderer instance) {
     if (r && r->world) lumx_world_set_materials(r->world, r->mats.data());
 }
 
-LumMeshModel lum_load_mesh_file(LumRenderer, const char*, bool) { return nullptr; }
+LumMeshModel lum_load_mesh_file(LumRenderer instance, const char* file, bool try_extract_palette) {
+    // loadModel(file) -> load_mesh (load_stuff.cpp:288-306): the first model;
+    // the file's Material table replaces the palette when asked and no
+    // palette was taken from a file before.
+    Renderer* r = get(instance);
+    if (!r || !r->ctx) return nullptr;
+    int size[3];
+    std::vector<uint8_t> v;
+    std::vector<float> mats(256 * 6);
+    if (!parse_vox_file(file, size, &v, mats.data())) return nullptr;
+    lumx_model_t m = nullptr;
+    if (lumx_model_create(r->ctx, v.data(), size[0], size[1], size[2], &m) != LUMX_OK) return nullptr;
+    r->models.push_back(m);
+    if (try_extract_palette && !r->has_palette) {
+        r->mats = mats;
+        r->has_palette = true;
+    }
+    return m;
+}
 
 LumMeshModel lum_load_mesh_data(LumRenderer instance, void* mesh_data, int x_size, int y_size, int z_size) {
     Renderer* r = get(instance);
@@ -505,6 +560,18 @@ void lumx_lum_set_gbuffer(LumRenderer instance, const uint8_t* matnorm_dev, cons
     r->gb_out = out_dev;
     r->gb_w = width;
     r->gb_h = height;
+}
+
+void lumx_lum_load_palette_file(LumRenderer instance, const char* file) {
+    // Renderer::loadPalette -> extract_palette (load_stuff.cpp:274-286).
+    Renderer* r = get(instance);
+    if (!r) return;
+    int size[3];
+    std::vector<uint8_t> v;
+    std::vector<float> mats(256 * 6);
+    if (!parse_vox_file(file, size, &v, mats.data())) return;
+    r->mats = mats;
+    r->has_palette = true;
 }
 
 void lumx_lum_set_radiance_shift(LumRenderer instance, int x, int y, int z) {
