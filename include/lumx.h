@@ -191,6 +191,16 @@ int lumx_map_models(lumx_world_t w, int n, const lumx_model_t* models, const lum
                     void* stream);
 int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models,
                            const lumx_placement* placements, void* stream);
+/* The same map with blockify and exec_copies on the device: the blockified
+ * world is built from a device copy of the origin world, so nothing
+ * proportional to the world crosses PCIe (only the model boxes).  Results
+ * equal lumx_map_models_placed bit for bit.  Host queries of the blockified
+ * state (lumx_world_get_blocks, lumx_world_palette_counter,
+ * lumx_select_probes) synchronise the device and read it back.  Slots beyond
+ * the palette are not allocated; lumx_world_palette_counter then returns
+ * LUMX_EOVERFLOW. */
+int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models,
+                           const lumx_placement* placements, void* stream);
 /* Palette slots allocated by the last map (the reference's paletteCounter). */
 int lumx_world_palette_counter(lumx_world_t w, int* out);
 

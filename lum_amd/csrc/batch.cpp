@@ -82,7 +82,8 @@ int launcher_create(lumx_context* ctx, int max_worlds, int ring_slots, Launcher*
     Launcher* L = new Launcher();
     L->ctx = ctx;
     L->max_worlds = max_worlds;
-    L->slot_bytes = round_up((size_t)max_worlds * record_bytes(), 256) + round_up((size_t)max_worlds * sizeof(int), 256);
+    L->slot_bytes = round_up((size_t)max_worlds * record_bytes(), 256) +
+                    2 * round_up(((size_t)max_worlds + 1) * sizeof(int), 256);
     L->ring.resize(ring_slots);
     L->capture_pool.resize(8);
     L->captured.reserve(64);
@@ -168,9 +169,11 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
     int rc = acquire_slot(L, s, &slot);
     if (rc) return rc;
     const size_t args_bytes = round_up((size_t)n * record_bytes(), 256);
+    const size_t begin_bytes = round_up(((size_t)n + 1) * sizeof(int), 256);
     RadianceArgs* args = (RadianceArgs*)slot->host;
     int* begin = (int*)((char*)slot->host + args_bytes);
-    long long total = 0;
+    int* pbegin = (int*)((char*)slot->host + args_bytes + begin_bytes);  // probe ranges of staged worlds
+    long long total = 0, total_staged = 0;
     for (int i = 0; i < n; ++i) {
         lumx_world* w = worlds[i];
         const int32_t* req = requests ? requests[i] : nullptr;
@@ -180,18 +183,20 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
         begin[i] = (int)total;
         total += round_up((size_t)args[i].n_probes * args[i].rays, (size_t)block);
         if (total > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batch too large");
-        const bool full = (req == nullptr && params[i].size == 1);
-        if (!full)
-            HIP_TRY(hipMemcpyAsync(args[i].field_out, args[i].field_in, (size_t)w->n_blocks * w->texel_bytes,
-                                   hipMemcpyDeviceToDevice, s));
+        pbegin[i] = (int)total_staged;
+        if (args[i].staged) total_staged += args[i].n_probes;
     }
-    HIP_TRY(hipMemcpyAsync(slot->dev, slot->host, args_bytes + (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_radiance_batch((const RadianceArgs*)slot->dev, (const int*)((char*)slot->dev + args_bytes), n,
-                                  (int)total, block, format, served, use_frame ? L->frame_dev : nullptr, s));
-    rc = release_slot(slot, s);
-    if (rc) return rc;
-    for (int i = 0; i < n; ++i) worlds[i]->cur ^= 1;
-    return LUMX_OK;
+    pbegin[n] = (int)total_staged;
+    HIP_TRY(hipMemcpyAsync(slot->dev, slot->host, args_bytes + 2 * begin_bytes, hipMemcpyHostToDevice, s));
+    const RadianceArgs* args_dev = (const RadianceArgs*)slot->dev;
+    HIP_TRY(launch_radiance_batch(args_dev, (const int*)((char*)slot->dev + args_bytes), n, (int)total, block, format,
+                                  served, use_frame ? L->frame_dev : nullptr, s));
+    // Subset updates land in their current fields in place.
+    HIP_TRY(launch_scatter_batch(args_dev, (const int*)((char*)slot->dev + args_bytes + begin_bytes), n,
+                                 (int)total_staged, (int)worlds[0]->texel_bytes, s));
+    for (int i = 0; i < n; ++i)
+        if (!args[i].staged) worlds[i]->cur ^= 1;  // full updates wrote the other field
+    return release_slot(slot, s);
 }
 
 int launcher_glossy(Launcher* L, lumx_world* const* worlds, int n, const uint8_t* const* matnorm,

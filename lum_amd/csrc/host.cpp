@@ -65,6 +65,8 @@ static lumx_radiance_tuning default_tuning() {
 // non-blocking stream is available through lumx_context_stream().
 static hipStream_t pick_stream(lumx_context*, void* s) { return (hipStream_t)s; }
 
+static int sync_host_state(lumx_world* w);
+
 #define REQUIRE_RESIDENT(w) \
     do { \
         if (!(w)->blocks) return fail(LUMX_EINVAL, "world is evicted (lumx_world_restore first)"); \
@@ -288,7 +290,11 @@ static hipError_t world_alloc_device(lumx_world* w) {
     chk(hipMalloc(&w->sel_bits, lumx::select_bits_bytes(w->n_blocks)));
     chk(hipMalloc(&w->sel_tiles, lumx::select_tiles(w->n_blocks) * sizeof(int)));
     chk(hipMalloc(&w->sel_claim, nb * sizeof(int)));
+    chk(hipMalloc(&w->origin_dev, nb * sizeof(int16_t)));
+    chk(hipMalloc(&w->blockify_state, 4 * sizeof(int)));
     if (e != hipSuccess) return e;
+    chk(hipMemset(w->blockify_state, 0, 4 * sizeof(int)));
+    w->origin_dirty = true;
     static_assert(sizeof(int) == 4, "claim words");
     chk(hipMemset(w->sel_claim, lumx::select_claim_idle() & 0xff, nb * sizeof(int)));
     chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
@@ -307,7 +313,7 @@ static size_t world_device_bytes(const lumx_world* w) {
     const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
     return nb * 2 + slots * 4096 + 256 * 6 * 4 + slots * 4 + 2 * nb * w->texel_bytes + ((nb + 31) / 32) * 4 +
            slots * 128 * 4 + 8 + lumx::select_bits_bytes(w->n_blocks) +
-           lumx::select_tiles(w->n_blocks) * sizeof(int) + nb * sizeof(int);
+           lumx::select_tiles(w->n_blocks) * sizeof(int) + nb * sizeof(int) + nb * 2 + 16;
 }
 
 static void world_free_device(lumx_world* w) {
@@ -323,6 +329,13 @@ static void world_free_device(lumx_world* w) {
     hipFree(w->sel_bits);
     hipFree(w->sel_tiles);
     hipFree(w->sel_claim);
+    hipFree(w->origin_dev);
+    hipFree(w->blockify_state);
+    hipFree(w->blockify_scratch);
+    w->origin_dev = nullptr;
+    w->blockify_state = nullptr;
+    w->blockify_scratch = nullptr;
+    w->blockify_scratch_bytes = 0;
     w->sel_bits = nullptr;
     w->sel_tiles = nullptr;
     w->sel_claim = nullptr;
@@ -340,6 +353,7 @@ static void world_free(lumx_world* w) {
     world_free_device(w);
     if (w->stage_blocks) hipHostFree(w->stage_blocks);
     if (w->stage_copy_src) hipHostFree(w->stage_copy_src);
+    if (w->stage_desc) hipHostFree(w->stage_desc);
     if (w->stage_done) hipEventDestroy(w->stage_done);
     for (void* p : w->graph_stage) hipHostFree(p);
     for (int p = 0; p < 3; ++p)
@@ -382,7 +396,7 @@ int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* 
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     chk(hipHostMalloc((void**)&w->stage_blocks, nb * sizeof(int16_t), hipHostMallocDefault));
     chk(hipEventCreateWithFlags(&w->stage_done, hipEventDisableTiming));
-    chk(hipHostMalloc((void**)&w->stage_copy_src, (size_t)slots * sizeof(int32_t), hipHostMallocDefault));
+    chk(hipHostMalloc((void**)&w->stage_copy_src, ((size_t)slots + 4) * sizeof(int32_t), hipHostMallocDefault));
     for (int p = 0; p < 3; ++p)
         for (int k = 0; k < 2; ++k) chk(hipEventCreate(&w->ev[p][k]));
     if (e != hipSuccess) {
@@ -423,11 +437,16 @@ int lumx_world_set_blocks(lumx_world_t w, const int16_t* blocks) {
     int rc = check_block_ids(w, blocks, w->n_blocks);
     if (rc) return rc;
     std::memcpy(w->origin.data(), blocks, (size_t)w->n_blocks * 2);
+    w->origin_dirty = true;
     return LUMX_OK;
 }
 
 int lumx_world_get_blocks(lumx_world_t w, int16_t* blocks, int blockified) {
     if (!w || !blocks) return fail(LUMX_EINVAL, "null argument");
+    if (blockified) {
+        int rc = sync_host_state(w);
+        if (rc) return rc;
+    }
     std::memcpy(blocks, blockified ? w->current.data() : w->origin.data(), (size_t)w->n_blocks * 2);
     return LUMX_OK;
 }
@@ -441,6 +460,7 @@ int lumx_world_set_block(lumx_world_t w, int x, int y, int z, int16_t b) {
     if (!in_world(w, x, y, z)) return fail(LUMX_EINVAL, "block (%d,%d,%d) outside the world", x, y, z);
     if (b < 0 || b >= w->desc.palette_slots) return fail(LUMX_EINVAL, "block id %d invalid", (int)b);
     w->origin[x + w->desc.size[0] * (y + w->desc.size[1] * z)] = b;
+    w->origin_dirty = true;
     return LUMX_OK;
 }
 
@@ -610,6 +630,27 @@ static int commit_blocks(lumx_world* w, hipStream_t s) {
         CopyArgs ca{w->palette, w->copy_src_dev, st, hi - st};
         HIP_TRY(launch_exec_copies(ca, s));
     }
+    // The device copy of the counter: the next device map's previous counter.
+    stage_copy[w->desc.palette_slots] = w->counter;
+    HIP_TRY(hipMemcpyAsync(w->blockify_state, stage_copy + w->desc.palette_slots, sizeof(int), hipMemcpyHostToDevice,
+                           s));
+    return LUMX_OK;
+}
+
+// After a device map the blockified world, the counter and the copy list live
+// on the device; host code that reads them brings them back first.
+static int sync_host_state(lumx_world* w) {
+    if (!w->host_stale) return LUMX_OK;
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    int state[4];
+    HIP_TRY(hipMemcpy(state, w->blockify_state, sizeof(state), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(w->current.data(), w->blocks, (size_t)w->n_blocks * 2, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(w->copy_src.data(), w->copy_src_dev, (size_t)w->desc.palette_slots * 4, hipMemcpyDeviceToHost));
+    w->counter = state[0];
+    w->prev_counter = state[0];
+    w->dev_overflow = state[2] != 0;
+    w->host_stale = false;
     return LUMX_OK;
 }
 
@@ -626,6 +667,8 @@ int lumx_world_upload_blocks(lumx_world_t w, void* stream) {
     if (!w) return fail(LUMX_EINVAL, "null world");
     REQUIRE_RESIDENT(w);
     HIP_TRY(hipSetDevice(w->ctx->device));
+    int rc0 = sync_host_state(w);
+    if (rc0) return rc0;
     w->current = w->origin;
     w->counter = w->desc.static_blocks;
     hipStream_t s = pick_stream(w->ctx, stream);
@@ -643,6 +686,8 @@ int lumx_world_evict(lumx_world_t w) {
     if (!w) return fail(LUMX_EINVAL, "null world");
     if (!w->blocks) return LUMX_OK;
     HIP_TRY(hipSetDevice(w->ctx->device));
+    int rc = sync_host_state(w);
+    if (rc) return rc;
     if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
     w->stage_pending = false;
     const size_t nb = (size_t)w->n_blocks;
@@ -675,6 +720,7 @@ int lumx_world_restore(lumx_world_t w) {
     HIP_TRY(hipMemcpy(w->mats, w->snap_mats.data(), 256 * 6 * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(w->field[w->cur], w->snap_field.data(), nb * w->texel_bytes, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(w->copy_src_dev, w->copy_src.data(), (size_t)w->desc.palette_slots * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(w->blockify_state, &w->counter, sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(launch_block_bits(w->blocks, w->n_blocks, w->empty_bits, nullptr));
     HIP_TRY(launch_palette_bits(w->palette, 0, used, w->pal_bits, nullptr));
     HIP_TRY(hipStreamSynchronize(nullptr));
@@ -695,7 +741,11 @@ int lumx_world_resident(lumx_world_t w, int* resident, size_t* device_bytes) {
 
 int lumx_world_palette_counter(lumx_world_t w, int* out) {
     if (!w || !out) return fail(LUMX_EINVAL, "null argument");
+    int rc = sync_host_state(w);
+    if (rc) return rc;
     *out = w->counter;
+    if (w->dev_overflow) return fail(LUMX_EOVERFLOW, "the last device map ran out of palette slots (%d)",
+                                     w->desc.palette_slots);
     return LUMX_OK;
 }
 
@@ -809,30 +859,9 @@ static int blockify(lumx_world* w, const lumx_placement& p) {
     return LUMX_OK;
 }
 
-int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
-                           void* stream) {
-    if (!w || n < 0 || (n > 0 && (!models || !pl))) return fail(LUMX_EINVAL, "bad argument");
-    for (int i = 0; i < n; ++i)
-        if (!models[i] || models[i]->ctx != w->ctx) return fail(LUMX_EINVAL, "model %d invalid", i);
-    REQUIRE_RESIDENT(w);
-    HIP_TRY(hipSetDevice(w->ctx->device));
-    hipStream_t s = pick_stream(w->ctx, stream);
-    // start_blockify: current = origin, counter = static.
-    w->current = w->origin;
-    w->counter = w->desc.static_blocks;
-    for (int i = 0; i < n; ++i) {
-        int rc = blockify(w, pl[i]);
-        if (rc) {
-            w->current = w->origin;
-            w->counter = w->desc.static_blocks;
-            return rc;
-        }
-    }
-    // end_blockify + exec_copies + map.
-    const bool capturing = stream_capturing(s);
-    if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));
-    int rc = commit_blocks(w, s);
-    if (rc) return rc;
+// map.comp for each placed model, in order, over the current device world.
+static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                         hipStream_t s) {
     const int st = w->desc.static_blocks;
     for (int i = 0; i < n; ++i) {
         const lumx_placement& p = pl[i];
@@ -850,6 +879,167 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, co
             for (int c = 0; c < 4; ++c) a.m[r * 4 + c] = p.minv[c * 4 + r];
         HIP_TRY(launch_map(a, s));
     }
+    return LUMX_OK;
+}
+
+// Block box of blockify_mesh for a placed model (internal_render.cpp:1032-1037).
+static void block_box(const lumx_world* w, const lumx_placement& p, int mn[3], int mx[3]) {
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = ((int)p.aabb_min[a] - 1) / 16;
+        mx[a] = ((int)p.aabb_max[a] + 1) / 16;
+        mn[a] = std::min(std::max(mn[a], 0), w->desc.size[a] - 1);
+        mx[a] = std::min(std::max(mx[a], 0), w->desc.size[a] - 1);
+    }
+}
+
+int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                           void* stream) {
+    if (!w || n < 0 || (n > 0 && (!models || !pl))) return fail(LUMX_EINVAL, "bad argument");
+    for (int i = 0; i < n; ++i)
+        if (!models[i] || models[i]->ctx != w->ctx) return fail(LUMX_EINVAL, "model %d invalid", i);
+    REQUIRE_RESIDENT(w);
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    hipStream_t s = pick_stream(w->ctx, stream);
+    const bool capturing = stream_capturing(s);
+    const int st = w->desc.static_blocks, slots = w->desc.palette_slots;
+    // Model boxes and their prefix over the concatenated element sequence.
+    std::vector<int> first(std::max(n, 1));
+    std::vector<int> box(8 * std::max(n, 1));  // per model: int4 min, int4 extent
+    long long total = 0;
+    for (int i = 0; i < n; ++i) {
+        int mn[3], mx[3];
+        block_box(w, pl[i], mn, mx);
+        first[i] = (int)total;
+        for (int a = 0; a < 3; ++a) {
+            box[8 * i + a] = mn[a];
+            box[8 * i + 4 + a] = mx[a] - mn[a] + 1;
+        }
+        box[8 * i + 3] = box[8 * i + 7] = 0;
+        total += (long long)box[8 * i + 4] * box[8 * i + 5] * box[8 * i + 6];
+    }
+    if (total > (1ll << 30)) return fail(LUMX_EINVAL, "model boxes cover too many blocks (%lld)", total);
+    const int n_el = (int)total;
+    const size_t tiles = blockify_tiles(n_el);
+    auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t off_box = align((size_t)std::max(n, 1) * 4);
+    const size_t off_bits = off_box + align((size_t)std::max(n, 1) * 32);
+    const size_t off_tiles = off_bits + align(tiles * (2048 / 8));
+    const size_t need = off_tiles + align(std::max<size_t>(tiles, 1) * 4);
+    if (need > w->blockify_scratch_bytes) {
+        if (capturing)
+            return fail(LUMX_EINVAL, "device map scratch too small inside a capture: run the same map once "
+                                     "outside the capture first");
+        HIP_TRY(hipStreamSynchronize(s));
+        hipFree(w->blockify_scratch);
+        w->blockify_scratch = nullptr;
+        w->blockify_scratch_bytes = 0;
+        HIP_TRY(hipMalloc(&w->blockify_scratch, need));
+        w->blockify_scratch_bytes = need;
+    }
+    // Staging of the model boxes (and of the origin world when it changed):
+    // pinned, shared by eager maps once the previous map's copies are done; a
+    // map recorded into a graph gets its own, read on every replay.
+    const size_t desc_bytes = off_bits;
+    const size_t origin_bytes = w->origin_dirty ? (size_t)w->n_blocks * 2 : 0;
+    char* stage = nullptr;
+    if (capturing) {
+        void* p = nullptr;
+        HIP_TRY(hipHostMalloc(&p, desc_bytes + origin_bytes + 16, hipHostMallocDefault));
+        w->graph_stage.push_back(p);
+        stage = (char*)p;
+    } else {
+        if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
+        w->stage_pending = false;
+        if (desc_bytes > w->stage_desc_bytes) {
+            if (w->stage_desc) hipHostFree(w->stage_desc);
+            w->stage_desc = nullptr;
+            w->stage_desc_bytes = 0;
+            HIP_TRY(hipHostMalloc(&w->stage_desc, desc_bytes, hipHostMallocDefault));
+            w->stage_desc_bytes = desc_bytes;
+        }
+        stage = (char*)w->stage_desc;
+    }
+    std::memcpy(stage, first.data(), (size_t)std::max(n, 1) * 4);
+    std::memcpy(stage + off_box, box.data(), (size_t)std::max(n, 1) * 32);
+    if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));
+    HIP_TRY(hipMemcpyAsync(w->blockify_scratch, stage, desc_bytes, hipMemcpyHostToDevice, s));
+    if (origin_bytes) {
+        int16_t* so = capturing ? (int16_t*)(stage + desc_bytes) : w->stage_blocks;
+        std::memcpy(so, w->origin.data(), origin_bytes);
+        HIP_TRY(hipMemcpyAsync(w->origin_dev, so, origin_bytes, hipMemcpyHostToDevice, s));
+        w->origin_dirty = false;
+    }
+    // start_blockify: the current world is the origin world.
+    HIP_TRY(hipMemcpyAsync(w->blocks, w->origin_dev, (size_t)w->n_blocks * 2, hipMemcpyDeviceToDevice, s));
+    BlockifyArgs a{};
+    a.origin = w->origin_dev;
+    a.blocks = w->blocks;
+    a.copy_src = w->copy_src_dev;
+    a.palette = w->palette;
+    a.claim = w->sel_claim;
+    a.claim_idle = select_claim_idle();
+    a.wx = w->desc.size[0];
+    a.wy = w->desc.size[1];
+    a.wz = w->desc.size[2];
+    a.static_count = st;
+    a.palette_slots = slots;
+    a.n_models = n;
+    a.n_elements = n_el;
+    a.n_tiles = (int)tiles;
+    char* base = (char*)w->blockify_scratch;
+    a.first = (const int*)base;
+    a.boxes = (const int4*)(base + off_box);
+    a.bits = (unsigned long long*)(base + off_bits);
+    a.tile_counts = (int*)(base + off_tiles);
+    a.state = w->blockify_state;
+    // Slots that may need copying: up to the larger of this map's and the
+    // previous map's counters.
+    const int prev_bound = w->host_stale ? w->dev_counter_bound : std::max(w->counter, w->prev_counter);
+    const int bound = (int)std::min<long long>(slots, std::max<long long>(st + total, prev_bound));
+    HIP_TRY(launch_blockify(a, bound - st, s));
+    HIP_TRY(launch_block_bits(w->blocks, w->n_blocks, w->empty_bits, s));
+    int rc = launch_models(w, n, models, pl, s);
+    if (rc) return rc;
+    if (bound > st) HIP_TRY(launch_palette_bits(w->palette, st, bound - st, w->pal_bits, s));
+    w->host_stale = true;
+    w->dev_counter_bound = (int)std::min<long long>(slots, st + total);
+    if (capturing) return LUMX_OK;
+    HIP_TRY(hipEventRecord(w->ev[1][1], s));
+    w->ev_valid[1] = true;
+    HIP_TRY(hipEventRecord(w->stage_done, s));
+    w->stage_pending = true;
+    return LUMX_OK;
+}
+
+int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                           void* stream) {
+    if (!w || n < 0 || (n > 0 && (!models || !pl))) return fail(LUMX_EINVAL, "bad argument");
+    for (int i = 0; i < n; ++i)
+        if (!models[i] || models[i]->ctx != w->ctx) return fail(LUMX_EINVAL, "model %d invalid", i);
+    REQUIRE_RESIDENT(w);
+    HIP_TRY(hipSetDevice(w->ctx->device));
+    int rc0 = sync_host_state(w);
+    if (rc0) return rc0;
+    w->dev_overflow = false;
+    hipStream_t s = pick_stream(w->ctx, stream);
+    // start_blockify: current = origin, counter = static.
+    w->current = w->origin;
+    w->counter = w->desc.static_blocks;
+    for (int i = 0; i < n; ++i) {
+        int rc = blockify(w, pl[i]);
+        if (rc) {
+            w->current = w->origin;
+            w->counter = w->desc.static_blocks;
+            return rc;
+        }
+    }
+    // end_blockify + exec_copies + map.
+    const bool capturing = stream_capturing(s);
+    if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));
+    int rc = commit_blocks(w, s);
+    if (rc) return rc;
+    rc = launch_models(w, n, models, pl, s);
+    if (rc) return rc;
     rc = commit_palette_bits(w, s);
     if (rc) return rc;
     if (capturing) return LUMX_OK;  // a graph re-reads the pinned staging on replay
@@ -929,27 +1119,30 @@ int lumx::radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx
     a.eb_words = (w->n_blocks + 31) / 32;
     a.pb_words = w->desc.palette_slots * 128;
     a.batch_counter = w->batch_counter;
+    a.staged = (req != nullptr || p->size != 1) ? 1 : 0;
     return LUMX_OK;
 }
 }  // extern "C++"
 
 // Launch one radiance update from field[cur] into field[cur ^ 1] with tuning t.
+// A full update (every probe) writes field[cur ^ 1], which then becomes
+// current.  A subset update writes probe i's new texel to field[cur ^ 1][i]
+// (free scratch between updates) and the scatter kernel then stores it in
+// field[cur] in place, so only the requested probes move and the current
+// field stays current.  apply = false (the autotuner) leaves out the scatter.
 static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p,
                            int32_t* keys, int32_t* steps, bool traced, hipStream_t s,
-                           const lumx_radiance_tuning& tune, const int32_t* count_dev = nullptr) {
+                           const lumx_radiance_tuning& tune, const int32_t* count_dev = nullptr,
+                           bool apply = true) {
     RadianceArgs a;
     int rc = radiance_args(w, req, n_req, p, &a);
     if (rc) return rc;
     a.n_requests_dev = count_dev;
     a.ray_keys = keys;
     a.ray_steps = steps;
-    // Probes outside the request list keep their value in the new field.
-    bool full = (req == nullptr && p->size == 1);
-    if (!full)
-        HIP_TRY(hipMemcpyAsync(a.field_out, a.field_in, (size_t)w->n_blocks * w->texel_bytes,
-                               hipMemcpyDeviceToDevice, s));
     if (tune.variant == 0 || tune.variant == 2) {
         HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, tune.variant == 2, s));
+        if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
         return LUMX_OK;
     }
     RadianceTune t;
@@ -964,13 +1157,14 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
         t.eb_lds = 1;
         base += 4 * (size_t)a.eb_words;
     }
-    const int used_slots = std::max(w->counter, w->prev_counter);
+    const int used_slots = w->host_stale ? w->dev_counter_bound : std::max(w->counter, w->prev_counter);
     if (tune.lds_palette_bits && base + 4 * (size_t)used_slots * 128 <= kLdsBudget) {
         t.pb_lds = 1;
         a.pb_words = used_slots * 128;
     }
     if (base > kLdsBudget) return fail(LUMX_EINVAL, "tiled radiance batch does not fit LDS (%zu B)", base);
     HIP_TRY(launch_radiance_tiled(a, w->desc.radiance_format, traced, t, w->ctx->n_cu, s));
+    if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
     return LUMX_OK;
 }
 
@@ -989,7 +1183,7 @@ static int radiance_impl(lumx_world* w, const int32_t* req, int n_req, const lum
         HIP_TRY(hipEventRecord(w->ev[0][1], s));
         w->ev_valid[0] = true;
     }
-    w->cur ^= 1;
+    if (req == nullptr && p->size == 1) w->cur ^= 1;  // a full update wrote the other field
     return LUMX_OK;
 }
 
@@ -1041,11 +1235,11 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
     lumx_radiance_tuning best_c = w->tune;
     for (const lumx_radiance_tuning& c : cands) {
         // Every trial writes field[cur ^ 1] only; the current field is unchanged.
-        rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c);
+        rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c, nullptr, false);
         if (rc) continue;  // shape does not fit this world
         HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < reps; ++r) {
-            rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c);
+            rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c, nullptr, false);
             if (rc) break;
         }
         HIP_TRY(hipEventRecord(e1, s));
@@ -1138,6 +1332,8 @@ int lumx_world_shift_radiance(lumx_world_t w, int dx, int dy, int dz, void* stre
 
 int lumx_select_probes(lumx_world_t w, const int32_t* special, int n_special, int32_t* out, int* n_out) {
     if (!w || !out || !n_out || n_special < 0 || (n_special > 0 && !special)) return fail(LUMX_EINVAL, "bad argument");
+    int rc0 = sync_host_state(w);
+    if (rc0) return rc0;
     const int wx = w->desc.size[0], wy = w->desc.size[1], wz = w->desc.size[2];
     const std::vector<int16_t>& cw = w->current;
     std::vector<uint8_t> set((size_t)w->n_blocks, 0);

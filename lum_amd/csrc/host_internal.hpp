@@ -67,7 +67,19 @@ struct lumx_world {
     unsigned int* batch_counter = nullptr;  // device: tiled kernel work counter (2 words)
     unsigned long long* sel_bits = nullptr; // device probe selection: flags, one word per 64 probes
     int* sel_tiles = nullptr;               // device probe selection: per-tile counts / offsets
-    int* sel_claim = nullptr;               // device probe selection: per probe special claim
+    int* sel_claim = nullptr;               // per block claim word shared by probe selection and
+                                            // device blockify (idle between calls)
+    // Device blockify.
+    int16_t* origin_dev = nullptr;          // device copy of the origin world
+    bool origin_dirty = true;               // origin changed since its last upload
+    int* blockify_state = nullptr;          // device: counter, copy bound, overflow
+    void* blockify_scratch = nullptr;       // device: model boxes, ballot words, tile counts
+    size_t blockify_scratch_bytes = 0;
+    bool host_stale = false;                // current/counter/copy_src live on the device
+    int dev_counter_bound = 0;              // upper bound of the device counter
+    bool dev_overflow = false;              // the last device map ran out of palette slots
+    void* stage_desc = nullptr;             // pinned staging of the device map's model boxes
+    size_t stage_desc_bytes = 0;
     int16_t* stage_blocks = nullptr;    // pinned host staging of the blockified world
     hipEvent_t stage_done = nullptr;    // recorded after the staging uploads of the last map
     bool stage_pending = false;

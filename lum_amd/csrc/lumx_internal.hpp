@@ -19,6 +19,8 @@ struct RadianceArgs {
     int log2_rays;          // log2(rays) when rays is a power of two, else -1
     int n_probes;           // dispatched probes (= n_requests / size)
     const int* n_requests_dev;  // device request count (nullptr = n_probes is exact)
+    int staged;             // 1: probe i's new texel goes to field_out[i] (a subset update,
+                            // applied in place by the scatter kernel), 0: to its own texel
     int time_seed;
     int size;
     int shift;
@@ -119,6 +121,27 @@ struct ShiftArgs {
     int extent[3];
 };
 
+// Device blockify (blockify.hip).
+struct BlockifyArgs {
+    const int16_t* origin;      // origin world (device)
+    int16_t* blocks;            // current world, a copy of origin on entry
+    int32_t* copy_src;          // per slot: static source block
+    uint8_t* palette;
+    int* claim;                 // per block: first model covering it, idle value otherwise
+    int claim_idle;
+    int wx, wy, wz;
+    int static_count, palette_slots;
+    int n_models, n_elements, n_tiles;
+    const int* first;           // per model: index of its first element (prefix of box volumes)
+    const int4* boxes;          // per model: {first block of its box}, {box extent in blocks}
+    unsigned long long* bits;   // one ballot word per 64 elements
+    int* tile_counts;           // per tile: count, then offset
+    int* state;                 // [0] counter of the last map, [1] copy bound, [2] overflow
+};
+
+size_t blockify_tiles(int n_elements);
+hipError_t launch_blockify(const BlockifyArgs& a, int copy_slots, hipStream_t stream);
+
 size_t select_tiles(int n_probes);
 size_t select_bits_bytes(int n_probes);
 int select_claim_idle();
@@ -126,6 +149,10 @@ hipError_t launch_select(const SelectArgs& a, hipStream_t stream);
 hipError_t launch_shift(const ShiftArgs& a, int texel_bytes, hipStream_t stream);
 
 hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, hipStream_t stream);
+// Applies a staged subset update: field_in[probe of i] = field_out[i].
+hipError_t launch_scatter(const RadianceArgs& a, int texel_bytes, hipStream_t stream);
+hipError_t launch_scatter_batch(const RadianceArgs* args_dev, const int* pbegin_dev, int n_worlds, int total_probes,
+                                int texel_bytes, hipStream_t stream);
 hipError_t launch_radiance_tiled(RadianceArgs a, int store, bool traced, const RadianceTune& t, int n_cu,
                                  hipStream_t stream);
 size_t radiance_tiled_lds_bytes(const RadianceArgs& a, const RadianceTune& t);

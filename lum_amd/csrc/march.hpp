@@ -57,6 +57,12 @@ struct WorldView {
     int block0_empty;  // palette slot 0 is all zero: skip its voxel fetch
 };
 
+// store_probe's image is A2B10G10R10 UNORM in the reference, whose store
+// turns a NaN component into 0; the float stores here keep that rule.
+LUMX_HD f3 unorm_nan_to_zero(f3 v) {
+    return {v.x != v.x ? 0.0f : v.x, v.y != v.y ? 0.0f : v.y, v.z != v.z ? 0.0f : v.z};
+}
+
 // Radiance texel storage formats.
 enum { STORE_F32 = 0, STORE_F16 = 1, STORE_BF16 = 2 };
 

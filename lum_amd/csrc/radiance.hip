@@ -10,6 +10,8 @@
 // in chunk order, then one division by R) so results do not depend on
 // timing.  Jacobi semantics: rays sample field_in, the probe is written to
 // field_out.
+#include <type_traits>
+
 #include "march.hpp"
 #include "lumx_internal.hpp"
 
@@ -145,7 +147,7 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
         old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
         const float al = a.alpha, bl = 1.0f - a.alpha;
         f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
-        Texel<STORE>::store(a.field_out, ti, nv);
+        Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
     }
 }
 
@@ -224,6 +226,70 @@ static hipError_t launch_lane(const RadianceArgs& a, int store, bool traced, hip
         case STORE_BF16: return launch_store<STORE_BF16, false, OCC>(a, stream);
         default: return launch_store<STORE_F32, false, OCC>(a, stream);
     }
+}
+
+// Scatter of a staged subset update: probe i of the launch (request
+// i * size + shift) gets field_out[i] in field_in, in place.  Probes outside
+// the world were not updated and are skipped, as in the update itself.
+template <int BYTES>
+__device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
+    typedef typename std::conditional<BYTES == 16, uint4, uint2>::type T;
+    if (i >= dispatched_probes(a)) return;
+    const WorldView& w = a.world;
+    int px, py, pz;
+    if (a.requests) {
+        const int4 q = a.requests[i * a.size + a.shift];
+        px = q.x;
+        py = q.y;
+        pz = q.z;
+    } else {
+        const int p = i * a.size + a.shift;
+        px = p % w.wx;
+        py = (p / w.wx) % w.wy;
+        pz = p / (w.wx * w.wy);
+    }
+    if ((unsigned)px >= (unsigned)w.wx || (unsigned)py >= (unsigned)w.wy || (unsigned)pz >= (unsigned)w.wz) return;
+    const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
+    ((T*)a.field_in)[ti] = ((const T*)a.field_out)[i];
+}
+
+template <int BYTES>
+__global__ void __launch_bounds__(256) scatter_kernel(RadianceArgs a) {
+    scatter_probe<BYTES>(a, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// Many worlds: world w owns probes [pbegin[w], pbegin[w + 1]).
+template <int BYTES>
+__global__ void __launch_bounds__(256) scatter_batch_kernel(const RadianceArgs* args, const int* pbegin, int n) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pbegin[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const RadianceArgs& a = args[lo];
+    if (!a.staged) return;
+    scatter_probe<BYTES>(a, t - pbegin[lo]);
+}
+
+hipError_t launch_scatter(const RadianceArgs& a, int texel_bytes, hipStream_t stream) {
+    if (a.n_probes <= 0) return hipSuccess;
+    const int grid = (a.n_probes + 255) / 256;
+    if (texel_bytes == 16) hipLaunchKernelGGL(scatter_kernel<16>, dim3(grid), dim3(256), 0, stream, a);
+    else hipLaunchKernelGGL(scatter_kernel<8>, dim3(grid), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_batch(const RadianceArgs* args_dev, const int* pbegin_dev, int n_worlds, int total_probes,
+                                int texel_bytes, hipStream_t stream) {
+    if (n_worlds <= 0 || total_probes <= 0) return hipSuccess;
+    const int grid = (total_probes + 255) / 256;
+    if (texel_bytes == 16)
+        hipLaunchKernelGGL(scatter_batch_kernel<16>, dim3(grid), dim3(256), 0, stream, args_dev, pbegin_dev, n_worlds);
+    else
+        hipLaunchKernelGGL(scatter_batch_kernel<8>, dim3(grid), dim3(256), 0, stream, args_dev, pbegin_dev, n_worlds);
+    return hipGetLastError();
 }
 
 // occ: march over the occupancy bits (global memory) instead of the u8 palette.
@@ -403,7 +469,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
             old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
             const float al = a.alpha, bl = 1.0f - a.alpha;
             f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
-            Texel<STORE>::store(a.field_out, ti, nv);
+            Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
         }
     }
     // The last workgroup to finish resets the batch counter for the next launch.

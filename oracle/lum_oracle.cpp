@@ -360,13 +360,15 @@ void radiance_probe(const World& w, const Field<T>& fin, T* fout, const int32_t*
     }
     V3<T> mean = reduce_mean<T>(lights, p.rays);
     const T* old = fin.rgba + 4 * ((size_t)px + (size_t)fin.wx * ((size_t)py + (size_t)fin.wy * pz));
-    V3<T> o = {std::min(std::max(old[0], (T)0), (T)1), std::min(std::max(old[1], (T)0), (T)1),
-               std::min(std::max(old[2], (T)0), (T)1)};
+    V3<T> o = {std::fmin(std::fmax(old[0], (T)0), (T)1), std::fmin(std::fmax(old[1], (T)0), (T)1),
+               std::fmin(std::fmax(old[2], (T)0), (T)1)};
     T a = (T)p.alpha;
     T* dst = fout + 4 * ((size_t)px + (size_t)fin.wx * ((size_t)py + (size_t)fin.wy * pz));
-    dst[0] = o.x * ((T)1 - a) + mean.x * a;
-    dst[1] = o.y * ((T)1 - a) + mean.y * a;
-    dst[2] = o.z * ((T)1 - a) + mean.z * a;
+    // A NaN component stores as 0, as the reference's UNORM image stores it.
+    auto unorm_nan = [](T v) { return v != v ? (T)0 : v; };
+    dst[0] = unorm_nan(o.x * ((T)1 - a) + mean.x * a);
+    dst[1] = unorm_nan(o.y * ((T)1 - a) + mean.y * a);
+    dst[2] = unorm_nan(o.z * ((T)1 - a) + mean.z * a);
     dst[3] = (T)1;
 }
 
@@ -732,7 +734,7 @@ void orc_diffuse_f32(const orc_world* w, const float* field, const uint8_t* matn
                            lm_proj[2 * 4 + r] * biased.z + lm_proj[3 * 4 + r];
                 float world_depth = 1.0f + c[2];
                 float u = (c[0] + 1.0f) / 2.0f, v = (c[1] + 1.0f) / 2.0f;
-                float dref = std::min(std::max(world_depth, 0.0f), 1.0f);
+                float dref = std::fmin(std::fmax(world_depth, 0.0f), 1.0f);
                 const int taps[5][2] = {{-1, 0}, {0, 0}, {1, 0}, {0, -1}, {0, 1}};
                 float total = 0.0f;
                 for (int k = 0; k < 5; ++k) {

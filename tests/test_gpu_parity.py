@@ -89,6 +89,31 @@ def test_radiance_all_probes(gpu_ctx, size, rays, seed, max_steps, tuning):
         w.close()
 
 
+@pytest.mark.parametrize("tuning", sorted(TUNINGS))
+def test_radiance_nan_stores_zero(gpu_ctx, tuning):
+    """NaN reaches a probe mean through a NaN field (or a 0 * inf ray); every
+    launch shape stores it as 0, like the reference's UNORM store, and
+    clamps an old NaN to 0: bit-exact with the oracle."""
+    import torch
+    size = (16, 16, 8)
+    blocks, pal, mats, field = make_case(size, 77)
+    field[..., :3] = np.nan
+    field[::3, ::2, 1::2, :3] = 0.25                   # a mix of NaN and finite texels
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    w.set_tuning(**TUNINGS[tuning])
+    try:
+        w.radiance_update(None, radiance_params(rays=64, time_seed=5, alpha=0.3, max_steps=64))
+        torch.cuda.synchronize()
+        got = w.read_radiance()
+        wd = orc.WorldData(blocks, full_palette(w, pal), mats)
+        ref = orc.radiance_update(wd, field, rays=64, time_seed=5, alpha=0.3, max_steps=64)
+        assert not np.isnan(ref).any() and not np.isnan(got).any()
+        assert np.array_equal(got, ref)
+        assert (got[..., :3] == 0).any() and (got[..., :3] > 0).any()
+    finally:
+        w.close()
+
+
 @pytest.mark.parametrize("tuning", ["lane", "tiled"])
 def test_radiance_request_list_jacobi(gpu_ctx, tuning):
     """Subset of probes with size/shift: unrequested probes keep their value."""

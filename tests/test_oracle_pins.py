@@ -287,6 +287,39 @@ def test_blend_is_linear_in_alpha_and_clamps_old():
     assert np.all(n1[..., 3] == 1)
 
 
+def test_axis_parallel_ray_into_solid_voxel_gives_nan_fraction():
+    """A direction with an exactly zero component whose refinement starts in
+    a solid voxel: mask 0 and tMax - tDelta = NaN on that axis (0 * inf), so
+    fraction = dot(NaN, 0) = NaN, as IEEE arithmetic in the shader gives."""
+    blocks, pal = solid_world(3, 3, 3)
+    w = world(blocks, pal)
+    f = field_of(3, 3, 3, value=0.5)
+    r = orc.trace_ray(w, f, [20.3, 21.7, 24.0], [0.98, 0.1990, 0.0])
+    assert r["hit"] and r["refine"] == 1 and np.isnan(r["fraction"])
+    assert np.all(np.isnan(r["light"]))
+
+
+def test_nan_never_stored():
+    """store_probe writes an A2B10G10R10 UNORM image in the reference, whose
+    conversion turns NaN into 0; the oracle's float store does the same.  An
+    all-NaN field: probes whose rays read it store 0; an old NaN value clamps
+    to 0 (fmax/fmin), so in an empty world the result equals a zero field's."""
+    rng = np.random.default_rng(8)
+    blocks, pal = solid_world(4, 4, 3)
+    blocks[2] = 0                       # air layer on top: some rays miss, most hit
+    mats = np.zeros((256, 6), np.float32)
+    mats[:, :3] = 0.5
+    w = world(blocks, pal, mats)
+    nan_f = field_of(4, 4, 3, value=np.nan)
+    out = orc.radiance_update(w, nan_f, rays=64, time_seed=1, alpha=0.2)
+    assert not np.isnan(out).any()
+    assert np.all(out[:2, ..., :3] == 0)   # buried probes: every ray hits and reads NaN
+    empty = world(np.zeros((2, 2, 2), np.int16), pal, mats)
+    a = orc.radiance_update(empty, field_of(2, 2, 2, value=np.nan), rays=32, time_seed=2, alpha=0.3)
+    b = orc.radiance_update(empty, field_of(2, 2, 2, value=0.0), rays=32, time_seed=2, alpha=0.3)
+    assert np.array_equal(a, b) and np.all(a[..., :3] > 0)
+
+
 @pytest.mark.parametrize("rays", [32, 64, 128])
 def test_uniform_field_fixed_point(rays):
     """Solid world of one material (colour c, emission e), uniform field L:
