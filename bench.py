@@ -23,6 +23,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs of this bench,
+    scaled by a 256 MiB copy), or None when no measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec.get("traffic_bytes")
+
+
 def radiance_bytes(keys, steps, n_probes, texel_bytes, request_bytes=0):
     """Gathered-bytes model of one radiance launch: 3 B (int16 block id +
     u8 voxel) per coarse sample and refinement fetch, 20 B of material plus
@@ -449,7 +462,8 @@ def run_serving(args):
                       "plain_launches": plain_st["launches"], "served_launches": served_st["launches"],
                       "plain_p50_ms": pct(plain_lat_s, 50), "served_p50_ms": pct(served_lat_s, 50)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("radiance_batch_kernel<1,false,false>"),
                      "kernel": "radiance_batch_kernel<1,false,false>", "kernel_ms": kern_ms,
                      "bytes_per_launch": bytes_per_launch,
                      "launch": "all %d worlds, every probe, 64 rays, FP16" % n_worlds},
