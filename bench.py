@@ -571,6 +571,20 @@ def run_chain(args):
     torch.cuda.synchronize()
     total = e0.elapsed_time(e1)
     per = {k: float(np.mean(v)) for k, v in pass_ms.items()}
+    # The same chain with blockify and exec_copies on the device.
+    dev_ms = {"map": [], "radiance": [], "glossy": []}
+    e0.record()
+    for k in range(args.steps):
+        w.map_models_device(ms, pls)
+        w.radiance_update(None, radiance_params(rays=64, time_seed=100 + k, alpha=0.1, max_steps=64))
+        w.glossy(mn_d, dp_d, cam, 128.0, 128, out=out)
+        torch.cuda.synchronize()
+        dev_ms["map"].append(w.last_pass_ms(1))
+        dev_ms["radiance"].append(w.last_pass_ms(0))
+        dev_ms["glossy"].append(w.last_pass_ms(2))
+    e1.record()
+    torch.cuda.synchronize()
+    dev_total = e0.elapsed_time(e1)
     rays = w.n_blocks * 64
     glossy_px = int((mats[mn[..., 0], 5] <= 0.5).sum())
     res = {
@@ -586,6 +600,8 @@ def run_chain(args):
                    "palette_slots_used": w.palette_counter(), "map_threads": map_threads,
                    "glossy_pixels": glossy_px},
         "pass_ms": per,
+        "device_map_chain": {"pass_ms": {k: float(np.mean(v)) for k, v in dev_ms.items()},
+                             "ms_per_step": dev_total / args.steps},
         "map_voxels_per_s": map_threads / (per["map"] / 1e3),
         "glossy_rays_per_s": glossy_px / (per["glossy"] / 1e3),
     }

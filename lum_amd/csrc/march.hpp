@@ -29,6 +29,23 @@
 // be exercised on the CPU from the same source.
 #define LUMX_HD __host__ __device__ __forceinline__
 
+// Every buffer the kernels read or write lives in global memory.  Pointers
+// taken from a descriptor in memory (the batched kernels) are otherwise
+// generic and compile to flat accesses; these helpers make them global.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LUMX_GLOBAL __attribute__((address_space(1)))
+#else
+#define LUMX_GLOBAL
+#endif
+template <class T>
+LUMX_HD T gld(const T* p, size_t i) {
+    return ((const LUMX_GLOBAL T*)p)[i];
+}
+template <class T>
+LUMX_HD void gst(T* p, size_t i, T v) {
+    ((LUMX_GLOBAL T*)p)[i] = v;
+}
+
 namespace lumx {
 
 struct f3 {
@@ -78,16 +95,16 @@ template <int STORE> struct Texel;
 template <> struct Texel<STORE_F32> {
     typedef float4 type;
     LUMX_HD static f3 load(const void* base, unsigned i) {
-        float4 v = ((const float4*)base)[i];
+        float4 v = gld((const float4*)base, i);
         return {v.x, v.y, v.z};
     }
     LUMX_HD static void store(void* base, unsigned i, f3 v) {
-        ((float4*)base)[i] = make_float4(v.x, v.y, v.z, 1.0f);
+        gst((float4*)base, i, make_float4(v.x, v.y, v.z, 1.0f));
     }
 };
 template <> struct Texel<STORE_F16> {
     LUMX_HD static f3 load(const void* base, unsigned i) {
-        uint2 raw = ((const uint2*)base)[i];
+        uint2 raw = gld((const uint2*)base, i);
         _Float16 h[4];
         __builtin_memcpy(h, &raw, 8);
         return {(float)h[0], (float)h[1], (float)h[2]};
@@ -96,12 +113,12 @@ template <> struct Texel<STORE_F16> {
         _Float16 h[4] = {(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)1.0f};
         uint2 raw;
         __builtin_memcpy(&raw, h, 8);
-        ((uint2*)base)[i] = raw;
+        gst((uint2*)base, i, raw);
     }
 };
 template <> struct Texel<STORE_BF16> {
     LUMX_HD static f3 load(const void* base, unsigned i) {
-        uint2 raw = ((const uint2*)base)[i];
+        uint2 raw = gld((const uint2*)base, i);
         return {bf16_to_f32((uint16_t)(raw.x & 0xffffu)), bf16_to_f32((uint16_t)(raw.x >> 16)),
                 bf16_to_f32((uint16_t)(raw.y & 0xffffu))};
     }
@@ -109,7 +126,7 @@ template <> struct Texel<STORE_BF16> {
         uint2 raw;
         raw.x = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
         raw.y = (uint32_t)f32_to_bf16(v.z) | ((uint32_t)0x3f80u << 16);
-        ((uint2*)base)[i] = raw;
+        gst((uint2*)base, i, raw);
     }
 };
 
@@ -128,7 +145,7 @@ LUMX_HD unsigned mul24(unsigned a, unsigned b) {
 // 32-bit vector offset.
 LUMX_HD int block_id(const WorldView& w, int bx, int by, int bz) {
     const unsigned lin = (unsigned)bx + mul24((unsigned)w.wx, (unsigned)by + mul24((unsigned)w.wy, (unsigned)bz));
-    return *(const int16_t*)((const char*)w.blocks + (lin << 1));
+    return gld((const int16_t*)((const char*)w.blocks + (lin << 1)), 0);
 }
 
 // voxel_in_palette: voxel (ix, iy, iz) mod 16 of block b.
@@ -136,7 +153,7 @@ LUMX_HD int voxel_in_block(const WorldView& w, int b, int ix, int iy, int iz) {
     if (b == 0 && w.block0_empty) return 0;
     if ((unsigned)b >= (unsigned)w.palette_slots) return 0;
     uint32_t off = ((uint32_t)b << 12) | (uint32_t)((ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8));
-    return w.palette[off];
+    return gld(w.palette, off);
 }
 
 // GetBlock + palette fetch for an integer voxel position.
@@ -154,11 +171,11 @@ struct MatF {
     float emit, rough;
 };
 LUMX_HD MatF get_mat(const WorldView& w, int v) {
-    const float* m = w.mats + v * 6;
+    const unsigned o = (unsigned)v * 6u;
     MatF r;
-    r.color = {m[0], m[1], m[2]};
-    r.emit = m[4];
-    r.rough = m[5];
+    r.color = {gld(w.mats, o), gld(w.mats, o + 1), gld(w.mats, o + 2)};
+    r.emit = gld(w.mats, o + 4);
+    r.rough = gld(w.mats, o + 5);
     return r;
 }
 
@@ -182,10 +199,11 @@ LUMX_HD f3 sample_texel(const void* field, int wx, int wy, int wz, f3 bp) {
     int x0 = wrap_i(ix, wx), x1 = wrap_i(ix + 1, wx);
     int y0 = wrap_i(iy, wy), y1 = wrap_i(iy + 1, wy);
     int z0 = wrap_i(iz, wz), z1 = wrap_i(iz + 1, wz);
-    const unsigned r00 = (unsigned)wx * ((unsigned)y0 + (unsigned)wy * (unsigned)z0);
-    const unsigned r10 = (unsigned)wx * ((unsigned)y1 + (unsigned)wy * (unsigned)z0);
-    const unsigned r01 = (unsigned)wx * ((unsigned)y0 + (unsigned)wy * (unsigned)z1);
-    const unsigned r11 = (unsigned)wx * ((unsigned)y1 + (unsigned)wy * (unsigned)z1);
+    // Operands below 2^24 (axes <= 4096, wy * wz <= 2^24): 24-bit multiplies.
+    const unsigned r00 = mul24((unsigned)wx, (unsigned)y0 + mul24((unsigned)wy, (unsigned)z0));
+    const unsigned r10 = mul24((unsigned)wx, (unsigned)y1 + mul24((unsigned)wy, (unsigned)z0));
+    const unsigned r01 = mul24((unsigned)wx, (unsigned)y0 + mul24((unsigned)wy, (unsigned)z1));
+    const unsigned r11 = mul24((unsigned)wx, (unsigned)y1 + mul24((unsigned)wy, (unsigned)z1));
     f3 t000 = Texel<STORE>::load(field, r00 + (unsigned)x0);
     f3 t100 = Texel<STORE>::load(field, r00 + (unsigned)x1);
     f3 t010 = Texel<STORE>::load(field, r10 + (unsigned)x0);
@@ -286,9 +304,9 @@ LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_
         st.x = (tMax.x <= tMax.y && tMax.x <= tMax.z) ? 1.0f : 0.0f;
         st.y = (tMax.x >= tMax.y && tMax.y <= tMax.z) ? 1.0f : 0.0f;
         st.z = (tMax.z <= tMax.x && tMax.z <= tMax.y) ? 1.0f : 0.0f;
-        vx += sx * (int)st.x;
-        vy += sy * (int)st.y;
-        vz += sz * (int)st.z;
+        vx += st.x != 0.0f ? sx : 0;  // sx * int(mask), without an integer multiply
+        vy += st.y != 0.0f ? sy : 0;
+        vz += st.z != 0.0f ? sz : 0;
         tMax = {tMax.x + tDelta.x * st.x, tMax.y + tDelta.y * st.y, tMax.z + tDelta.z * st.z};
         cv = get_voxel(w, vx, vy, vz, &blk);
         r.refine++;
@@ -327,8 +345,8 @@ struct OccView {
         int bx = ix / 16, by = iy / 16, bz = iz / 16;
         if ((unsigned)bx >= (unsigned)wx || (unsigned)by >= (unsigned)wy || (unsigned)bz >= (unsigned)wz) return 0;
         unsigned lin = (unsigned)(bx + wx * (by + wy * bz));
-        if ((eb[lin >> 5] >> (lin & 31)) & 1u) return 0;
-        return ids[lin];
+        if ((eb[lin >> 5] >> (lin & 31)) & 1u) return 0;  // eb and pb may point into LDS
+        return gld(ids, lin);
     }
     LUMX_HD bool occupied(int b, int ix, int iy, int iz) const {
         if (b == 0 && block0_empty) return false;
@@ -338,7 +356,7 @@ struct OccView {
     }
     LUMX_HD int voxel_value(int b, int ix, int iy, int iz) const {
         int idx = (ix & 15) | ((iy & 15) << 4) | ((iz & 15) << 8);
-        return palette[((uint32_t)b << 12) | (uint32_t)idx];
+        return gld(palette, ((uint32_t)b << 12) | (uint32_t)idx);
     }
 };
 
@@ -402,9 +420,9 @@ LUMX_HD RayHit cast_ray_occ(const OccView& w, f3 o, f3 d, float max_dist, int ma
         st.x = (tMax.x <= tMax.y && tMax.x <= tMax.z) ? 1.0f : 0.0f;
         st.y = (tMax.x >= tMax.y && tMax.y <= tMax.z) ? 1.0f : 0.0f;
         st.z = (tMax.z <= tMax.x && tMax.z <= tMax.y) ? 1.0f : 0.0f;
-        vx += sx * (int)st.x;
-        vy += sy * (int)st.y;
-        vz += sz * (int)st.z;
+        vx += st.x != 0.0f ? sx : 0;  // sx * int(mask), without an integer multiply
+        vy += st.y != 0.0f ? sy : 0;
+        vz += st.z != 0.0f ? sz : 0;
         tMax = {tMax.x + tDelta.x * st.x, tMax.y + tDelta.y * st.y, tMax.z + tDelta.z * st.z};
         vb = w.block_at(vx, vy, vz);
         cv = w.occupied(vb, vx, vy, vz);

@@ -36,7 +36,7 @@ __device__ __forceinline__ void hash33(uint32_t x, uint32_t y, uint32_t z, f3* o
 // the device (the reference's wg_count = n_requests / size).
 __device__ __forceinline__ int dispatched_probes(const RadianceArgs& a) {
     if (!a.n_requests_dev) return a.n_probes;
-    return min(a.n_probes, *a.n_requests_dev / a.size);
+    return min(a.n_probes, gld(a.n_requests_dev, 0) / a.size);
 }
 
 // One ray per thread; tid is the thread's index within this world's launch
@@ -56,7 +56,7 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
     f3 light = {0.0f, 0.0f, 0.0f};
     if (active) {
         if (a.requests) {
-            int4 q = a.requests[wg * a.size + a.shift];
+            int4 q = gld(a.requests, (size_t)(wg * a.size + a.shift));
             px = q.x;
             py = q.y;
             pz = q.z;
@@ -73,7 +73,7 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
         f3 r0, r1;
         hash33(gidx + t, t, t, &r0);
         hash33(gidx + t + 1u, t, t, &r1);
-        const float4 fb = a.fib[lane];
+        const float4 fb = gld(a.fib, (size_t)lane);
         f3 corner = f3{(float)px, (float)py, (float)pz} * 16.0f;
         f3 center = corner + f3{8.0f, 8.0f, 8.0f};
         const f3 half = {0.5f, 0.5f, 0.5f};
@@ -113,10 +113,10 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
         }
         if (TRACED) {
             size_t ri = (size_t)wg * R + lane;
-            if (a.ray_keys) a.ray_keys[ri] = h.hit ? h.key : 0;
+            if (a.ray_keys) gst(a.ray_keys, ri, h.hit ? h.key : 0);
             if (a.ray_steps) {
-                a.ray_steps[2 * ri + 0] = h.coarse;
-                a.ray_steps[2 * ri + 1] = h.refine;
+                gst(a.ray_steps, 2 * ri + 0, h.coarse);
+                gst(a.ray_steps, 2 * ri + 1, h.refine);
             }
         }
     }
@@ -238,7 +238,7 @@ __device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
     const WorldView& w = a.world;
     int px, py, pz;
     if (a.requests) {
-        const int4 q = a.requests[i * a.size + a.shift];
+        const int4 q = gld(a.requests, (size_t)(i * a.size + a.shift));
         px = q.x;
         py = q.y;
         pz = q.z;
@@ -250,7 +250,7 @@ __device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
     }
     if ((unsigned)px >= (unsigned)w.wx || (unsigned)py >= (unsigned)w.wy || (unsigned)pz >= (unsigned)w.wz) return;
     const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
-    ((T*)a.field_in)[ti] = ((const T*)a.field_out)[i];
+    gst((T*)a.field_in, ti, gld((const T*)a.field_out, (size_t)i));
 }
 
 template <int BYTES>
