@@ -967,7 +967,8 @@ int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, co
         int16_t* so = capturing ? (int16_t*)(stage + desc_bytes) : w->stage_blocks;
         std::memcpy(so, w->origin.data(), origin_bytes);
         HIP_TRY(hipMemcpyAsync(w->origin_dev, so, origin_bytes, hipMemcpyHostToDevice, s));
-        w->origin_dirty = false;
+        // A recorded upload runs only when the graph does: eager maps keep uploading.
+        if (!capturing) w->origin_dirty = false;
     }
     // start_blockify: the current world is the origin world.
     HIP_TRY(hipMemcpyAsync(w->blocks, w->origin_dev, (size_t)w->n_blocks * 2, hipMemcpyDeviceToDevice, s));
@@ -1120,6 +1121,8 @@ int lumx::radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx
     a.pb_words = w->desc.palette_slots * 128;
     a.batch_counter = w->batch_counter;
     a.staged = (req != nullptr || p->size != 1) ? 1 : 0;
+    a.div_x = make_divisor((uint32_t)w->desc.size[0]);
+    a.div_xy = make_divisor((uint32_t)w->desc.size[0] * (uint32_t)w->desc.size[1]);
     return LUMX_OK;
 }
 }  // extern "C++"

@@ -19,6 +19,7 @@ struct RadianceArgs {
     int log2_rays;          // log2(rays) when rays is a power of two, else -1
     int n_probes;           // dispatched probes (= n_requests / size)
     const int* n_requests_dev;  // device request count (nullptr = n_probes is exact)
+    Divisor div_x, div_xy;  // wx and wx * wy, for probe coordinates of x-fastest indices
     int staged;             // 1: probe i's new texel goes to field_out[i] (a subset update,
                             // applied in place by the scatter kernel), 0: to its own texel
     int time_seed;

@@ -95,16 +95,17 @@ template <int STORE> struct Texel;
 template <> struct Texel<STORE_F32> {
     typedef float4 type;
     LUMX_HD static f3 load(const void* base, unsigned i) {
-        float4 v = gld((const float4*)base, i);
+        // 32-bit byte offsets: a field holds at most 2^28 texels of 16 B.
+        float4 v = gld((const float4*)((const char*)base + (i << 4)), 0);
         return {v.x, v.y, v.z};
     }
     LUMX_HD static void store(void* base, unsigned i, f3 v) {
-        gst((float4*)base, i, make_float4(v.x, v.y, v.z, 1.0f));
+        gst((float4*)((char*)base + (i << 4)), 0, make_float4(v.x, v.y, v.z, 1.0f));
     }
 };
 template <> struct Texel<STORE_F16> {
     LUMX_HD static f3 load(const void* base, unsigned i) {
-        uint2 raw = gld((const uint2*)base, i);
+        uint2 raw = gld((const uint2*)((const char*)base + (i << 3)), 0);
         _Float16 h[4];
         __builtin_memcpy(h, &raw, 8);
         return {(float)h[0], (float)h[1], (float)h[2]};
@@ -113,12 +114,12 @@ template <> struct Texel<STORE_F16> {
         _Float16 h[4] = {(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)1.0f};
         uint2 raw;
         __builtin_memcpy(&raw, h, 8);
-        gst((uint2*)base, i, raw);
+        gst((uint2*)((char*)base + (i << 3)), 0, raw);
     }
 };
 template <> struct Texel<STORE_BF16> {
     LUMX_HD static f3 load(const void* base, unsigned i) {
-        uint2 raw = gld((const uint2*)base, i);
+        uint2 raw = gld((const uint2*)((const char*)base + (i << 3)), 0);
         return {bf16_to_f32((uint16_t)(raw.x & 0xffffu)), bf16_to_f32((uint16_t)(raw.x >> 16)),
                 bf16_to_f32((uint16_t)(raw.y & 0xffffu))};
     }
@@ -126,7 +127,7 @@ template <> struct Texel<STORE_BF16> {
         uint2 raw;
         raw.x = (uint32_t)f32_to_bf16(v.x) | ((uint32_t)f32_to_bf16(v.y) << 16);
         raw.y = (uint32_t)f32_to_bf16(v.z) | ((uint32_t)0x3f80u << 16);
-        gst((uint2*)base, i, raw);
+        gst((uint2*)((char*)base + (i << 3)), 0, raw);
     }
 };
 
@@ -136,6 +137,32 @@ LUMX_HD unsigned mul24(unsigned a, unsigned b) {
     return __umul24(a, b);
 #else
     return a * b;
+#endif
+}
+
+// Division of n < 2^28 by an invariant d in [1, 2^24] as a multiply-high
+// and a shift (division by invariant integers, Granlund & Montgomery 1994):
+// k = max(32, 28 + ceil(log2 d)), m = ceil(2^k / d), n / d = (n * m) >> k.
+// With e = m * d - 2^k < d, n * e < 2^28 * d <= 2^k keeps every quotient
+// exact.  d = 1 is m = 0 (n itself).
+struct Divisor {
+    uint32_t m;
+    uint32_t shift;  // k - 32
+};
+inline Divisor make_divisor(uint32_t d) {
+    if (d <= 1) return {0u, 0u};
+    uint32_t L = 0;
+    while ((1ull << L) < d) ++L;
+    const uint32_t k = 28 + L > 32 ? 28 + L : 32;
+    const unsigned long long m = ((1ull << k) + d - 1) / d;
+    return {(uint32_t)m, k - 32};
+}
+LUMX_HD uint32_t fast_div(uint32_t n, Divisor v) {
+    if (v.m == 0) return n;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(n, v.m) >> v.shift;
+#else
+    return (uint32_t)(((unsigned long long)n * v.m) >> 32) >> v.shift;
 #endif
 }
 

@@ -32,6 +32,25 @@ __device__ __forceinline__ void hash33(uint32_t x, uint32_t y, uint32_t z, f3* o
     *out = {(float)x * inv, (float)y * inv, (float)z * inv};
 }
 
+// total / R.  For R a power of two the quotient is total scaled by 2^-k,
+// which the product with the exact 1/R rounds identically; other R divide.
+__device__ __forceinline__ f3 probe_mean(f3 total, int R) {
+    if ((R & (R - 1)) == 0) {
+        const float inv = 1.0f / (float)R;
+        return total * inv;
+    }
+    return total / (float)R;
+}
+
+// Coordinates of probe p of the x-fastest order (wx * wy * wz < 2^28).
+__device__ __forceinline__ void probe_xyz(const RadianceArgs& a, uint32_t p, int* x, int* y, int* z) {
+    const uint32_t q = fast_div(p, a.div_x);    // p / wx
+    const uint32_t r = fast_div(p, a.div_xy);   // p / (wx * wy)
+    *x = (int)(p - mul24(q, (uint32_t)a.world.wx));
+    *y = (int)(q - mul24(r, (uint32_t)a.world.wy));
+    *z = (int)r;
+}
+
 // Probes of this launch: n_probes, or fewer when the request count lives on
 // the device (the reference's wg_count = n_requests / size).
 __device__ __forceinline__ int dispatched_probes(const RadianceArgs& a) {
@@ -61,9 +80,7 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
             py = q.y;
             pz = q.z;
         } else {
-            px = wg % w.wx;
-            py = (wg / w.wx) % w.wy;
-            pz = wg / (w.wx * w.wy);
+            probe_xyz(a, (uint32_t)wg, &px, &py, &pz);
         }
         valid = (unsigned)px < (unsigned)w.wx && (unsigned)py < (unsigned)w.wy && (unsigned)pz < (unsigned)w.wz;
     }
@@ -141,7 +158,7 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
         }
     }
     if (valid && lane == 0) {
-        f3 mean = total / (float)R;
+        f3 mean = probe_mean(total, R);
         const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
         f3 old = Texel<STORE>::load(a.field_in, ti);
         old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
@@ -244,9 +261,7 @@ __device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
         pz = q.z;
     } else {
         const int p = i * a.size + a.shift;
-        px = p % w.wx;
-        py = (p / w.wx) % w.wy;
-        pz = p / (w.wx * w.wy);
+        probe_xyz(a, (uint32_t)p, &px, &py, &pz);
     }
     if ((unsigned)px >= (unsigned)w.wx || (unsigned)py >= (unsigned)w.wy || (unsigned)pz >= (unsigned)w.wz) return;
     const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
@@ -376,9 +391,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
                     py = rq.y;
                     pz = rq.z;
                 } else {
-                    px = wg % wv.wx;
-                    py = (wg / wv.wx) % wv.wy;
-                    pz = wg / (wv.wx * wv.wy);
+                    probe_xyz(a, (uint32_t)wg, &px, &py, &pz);
                 }
                 valid = (unsigned)px < (unsigned)wv.wx && (unsigned)py < (unsigned)wv.wy &&
                         (unsigned)pz < (unsigned)wv.wz;
@@ -452,9 +465,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
                 py = rq.y;
                 pz = rq.z;
             } else {
-                px = wg % wv.wx;
-                py = (wg / wv.wx) % wv.wy;
-                pz = wg / (wv.wx * wv.wy);
+                probe_xyz(a, (uint32_t)wg, &px, &py, &pz);
             }
             if ((unsigned)px >= (unsigned)wv.wx || (unsigned)py >= (unsigned)wv.wy || (unsigned)pz >= (unsigned)wv.wz)
                 continue;
@@ -463,7 +474,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
                 const int q = c * C * P + p;
                 total = total + f3{s_lx[q], s_ly[q], s_lz[q]};
             }
-            f3 mean = total / (float)R;
+            f3 mean = probe_mean(total, R);
             const unsigned ti = (unsigned)px + (unsigned)wv.wx * ((unsigned)py + (unsigned)wv.wy * (unsigned)pz);
             f3 old = Texel<STORE>::load(a.field_in, ti);
             old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};

@@ -62,4 +62,10 @@ void mh_sample_radiance(const float* field, int wx, int wy, int wz, int n, const
     }
 }
 
+// n[i] / d through the kernels' multiply-high division.
+void mh_fast_div(const uint32_t* n, int count, uint32_t d, uint32_t* out) {
+    const Divisor v = make_divisor(d);
+    for (int i = 0; i < count; ++i) out[i] = fast_div(n[i], v);
+}
+
 }  // extern "C"

@@ -85,6 +85,26 @@ def test_host_built_cast_ray_matches_oracle(mh, occ, seed, size, max_steps):
     assert 0.2 * n < hits < n
 
 
+def test_fast_division_is_exact_below_2_pow_28(mh):
+    """Probe coordinates use n / d as a multiply-high and a shift: exact for
+    every n < 2^28 and d in [1, 2^24] (world axes up to 4096, wx * wy up to
+    2^24).  Checked on every power of two and its neighbours, the world sizes
+    of the workloads and random divisors, with random and edge dividends."""
+    rng = np.random.default_rng(3)
+    ds = {1, 2, 3, 5, 7, 48, 48 * 48, 256, 256 * 256, 4096, 4096 * 4096, 130, 130 * 9, 1 << 24}
+    for k in range(1, 25):
+        ds.update({(1 << k) - 1, 1 << k, (1 << k) + 1})
+    ds.update(int(v) for v in rng.integers(1, 1 << 24, 300))
+    top = (1 << 28) - 1
+    for d in sorted(x for x in ds if 1 <= x <= (1 << 24)):
+        n = np.concatenate([rng.integers(0, 1 << 28, 2000), np.arange(0, 64), top - np.arange(64),
+                            (np.arange(1, 200) * d) - 1, np.arange(1, 200) * d]).astype(np.uint64)
+        n = n[n <= top].astype(np.uint32)
+        out = np.zeros_like(n)
+        mh.mh_fast_div(_p(n), len(n), C.c_uint32(d), _p(out))
+        assert np.array_equal(out, n // np.uint32(d)), d
+
+
 def test_host_built_sample_radiance_matches_oracle(mh):
     rng = np.random.default_rng(9)
     size = (7, 5, 4)
