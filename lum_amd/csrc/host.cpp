@@ -438,6 +438,7 @@ int lumx_world_set_blocks(lumx_world_t w, const int16_t* blocks) {
     if (rc) return rc;
     std::memcpy(w->origin.data(), blocks, (size_t)w->n_blocks * 2);
     w->origin_dirty = true;
+    w->origin_ids_dirty = true;
     return LUMX_OK;
 }
 
@@ -461,6 +462,7 @@ int lumx_world_set_block(lumx_world_t w, int x, int y, int z, int16_t b) {
     if (b < 0 || b >= w->desc.palette_slots) return fail(LUMX_EINVAL, "block id %d invalid", (int)b);
     w->origin[x + w->desc.size[0] * (y + w->desc.size[1] * z)] = b;
     w->origin_dirty = true;
+    w->origin_ids_dirty = true;
     return LUMX_OK;
 }
 
@@ -860,12 +862,30 @@ static int blockify(lumx_world* w, const lumx_placement& p) {
 }
 
 // map.comp for each placed model, in order, over the current device world.
+// Models go in layers: a model's layer is one more than that of the last
+// earlier model whose write box (the voxels its dispatch covers) intersects
+// its own.  Models of one layer write disjoint palette bytes, so a layer is
+// one launch (in groups of kMapGroup) and the result equals submission-order
+// mapping.  An origin world holding dynamic ids could alias two blocks to one
+// slot; it maps one model per launch.
 static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
                          hipStream_t s) {
     const int st = w->desc.static_blocks;
+    std::vector<MapArgs> args(std::max(n, 1));
+    std::vector<int> layer(std::max(n, 1), 0);
+    if (w->origin_ids_dirty) {
+        w->origin_has_dynamic = false;
+        for (int16_t b : w->origin)
+            if (b >= st) {
+                w->origin_has_dynamic = true;
+                break;
+            }
+        w->origin_ids_dirty = false;
+    }
+    const bool aliasing = w->origin_has_dynamic;
     for (int i = 0; i < n; ++i) {
         const lumx_placement& p = pl[i];
-        MapArgs a;
+        MapArgs& a = args[i];
         a.blocks = w->blocks;
         a.palette = w->palette;
         a.model = models[i]->voxels;
@@ -877,7 +897,38 @@ static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const
         a.shift_x = p.shift[0]; a.shift_y = p.shift[1]; a.shift_z = p.shift[2];
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 4; ++c) a.m[r * 4 + c] = p.minv[c * 4 + r];
-        HIP_TRY(launch_map(a, s));
+        layer[i] = 0;
+        for (int j = 0; j < i; ++j) {
+            const MapArgs& b = args[j];
+            const bool overlap = aliasing ||
+                (a.shift_x < b.shift_x + b.nx && b.shift_x < a.shift_x + a.nx && a.shift_y < b.shift_y + b.ny &&
+                 b.shift_y < a.shift_y + a.ny && a.shift_z < b.shift_z + b.nz && b.shift_z < a.shift_z + a.nz);
+            if (overlap) layer[i] = std::max(layer[i], layer[j] + 1);
+        }
+    }
+    const int n_layers = n ? *std::max_element(layer.begin(), layer.begin() + n) + 1 : 0;
+    for (int l = 0; l < n_layers; ++l) {
+        MapGroup g;
+        g.n = 0;
+        g.begin[0] = 0;
+        for (int i = 0; i <= n; ++i) {
+            const bool take = i < n && layer[i] == l;
+            if (take) {
+                const long long threads = (long long)args[i].nx * args[i].ny * args[i].nz;
+                if (threads <= 0) continue;
+                const long long groups = (threads + 255) / 256;
+                if ((long long)g.begin[g.n] + groups > (1ll << 31) - 1)
+                    return fail(LUMX_EINVAL, "model %d dispatch too large", i);
+                g.m[g.n] = args[i];
+                g.begin[g.n + 1] = g.begin[g.n] + (int)groups;
+                g.n++;
+            }
+            if (g.n == kMapGroup || (i == n && g.n > 0)) {
+                HIP_TRY(launch_map_group(g, s));
+                g.n = 0;
+                g.begin[0] = 0;
+            }
+        }
     }
     return LUMX_OK;
 }

@@ -72,6 +72,8 @@ struct lumx_world {
     // Device blockify.
     int16_t* origin_dev = nullptr;          // device copy of the origin world
     bool origin_dirty = true;               // origin changed since its last upload
+    bool origin_ids_dirty = true;           // origin changed since origin_has_dynamic was computed
+    bool origin_has_dynamic = false;        // an origin id >= static_blocks (a slot two blocks may share)
     int* blockify_state = nullptr;          // device: counter, copy bound, overflow
     void* blockify_scratch = nullptr;       // device: model boxes, ballot words, tile counts
     size_t blockify_scratch_bytes = 0;

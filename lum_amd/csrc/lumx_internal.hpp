@@ -62,6 +62,14 @@ struct MapArgs {
     float m[12];            // rows 0..2 of the inverse transform: m[r*4 + c]
 };
 
+// Models of one map launch (kernel argument, at most kMapGroup models).
+constexpr int kMapGroup = 16;
+struct MapGroup {
+    MapArgs m[kMapGroup];
+    int begin[kMapGroup + 1];  // workgroup ranges of 256 threads, begin[n] = grid size
+    int n;
+};
+
 struct CopyArgs {
     uint8_t* palette;
     const int32_t* copy_src;  // per slot: static source block (0 = zero fill)
@@ -161,6 +169,7 @@ hipError_t launch_block_bits(const int16_t* blocks, int n_blocks, uint32_t* bits
 hipError_t launch_palette_bits(const uint8_t* palette, int first_slot, int n_slots, uint32_t* bits,
                                hipStream_t stream);
 hipError_t launch_map(const MapArgs& a, hipStream_t stream);
+hipError_t launch_map_group(const MapGroup& g, hipStream_t stream);
 hipError_t launch_exec_copies(const CopyArgs& a, hipStream_t stream);
 hipError_t launch_glossy(const GlossyArgs& a, int store, bool traced, hipStream_t stream);
 hipError_t launch_diffuse(const DiffuseArgs& a, int store, hipStream_t stream);

@@ -14,10 +14,8 @@
 
 namespace lumx {
 
-__global__ void __launch_bounds__(256) map_kernel(MapArgs a) {
-    const long long total = (long long)a.nx * a.ny * a.nz;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
+__device__ __forceinline__ void map_thread(const MapArgs& a, long long t) {
+    if (t >= (long long)a.nx * a.ny * a.nz) return;
     const int gx = (int)(t % a.nx);
     const int gy = (int)((t / a.nx) % a.ny);
     const int gz = (int)(t / ((long long)a.nx * a.ny));
@@ -51,6 +49,20 @@ __global__ void __launch_bounds__(256) map_kernel(MapArgs a) {
     a.palette[off] = v;
 }
 
+__global__ void __launch_bounds__(256) map_kernel(MapArgs a) {
+    map_thread(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Several models whose write boxes are disjoint in one launch: model i owns
+// workgroups [begin[i], begin[i + 1]).  Disjoint boxes write disjoint palette
+// bytes, so the result equals mapping them one after another.
+__global__ void __launch_bounds__(256) map_group_kernel(MapGroup g) {
+    const int b = blockIdx.x;
+    int i = 0;
+    while (i + 1 < g.n && g.begin[i + 1] <= b) ++i;
+    map_thread(g.m[i], (long long)(b - g.begin[i]) * 256 + threadIdx.x);
+}
+
 // One workgroup of 256 threads per slot, 16 B per thread.
 __global__ void __launch_bounds__(256) copy_kernel(CopyArgs a) {
     const int slot = a.first_slot + blockIdx.x;
@@ -66,6 +78,12 @@ hipError_t launch_map(const MapArgs& a, hipStream_t stream) {
     if (total <= 0) return hipSuccess;
     int grid = (int)((total + 255) / 256);
     hipLaunchKernelGGL(map_kernel, dim3(grid), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_group(const MapGroup& g, hipStream_t stream) {
+    if (g.n <= 0 || g.begin[g.n] <= 0) return hipSuccess;
+    hipLaunchKernelGGL(map_group_kernel, dim3(g.begin[g.n]), dim3(256), 0, stream, g);
     return hipGetLastError();
 }
 
