@@ -247,6 +247,7 @@ def run_batched(args):
 
     for c in cases:
         c.params.time_seed = 0
+    batch_block, _ = batch.autotune(params, reps=3)
     for k in range(args.warmup):
         plain_frame(k)
         batched_frame(k)
@@ -306,6 +307,7 @@ def run_batched(args):
         "plain_value": plain_value,
         "speedup_vs_plain": value / plain_value,
         "per_frame": per_frame,
+        "batch_block": batch_block,
     }
     batch.close()
     for c in cases:

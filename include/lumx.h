@@ -280,6 +280,15 @@ int lumx_batch_destroy(lumx_batch_t b);
  * result equals its own lumx_radiance_update with the same arguments. */
 int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests_dev, const int* n_requests,
                         const lumx_radiance_params* params, void* stream);
+/* Autotuner of the batched radiance launch: times workgroups of 256, 512
+ * and 1024 threads on every probe of every world with params[i] per world
+ * (the worlds' fields are left unchanged), keeps the fastest for this
+ * batch's later launches and returns it.  No reference counterpart. */
+int lumx_batch_autotune(lumx_batch_t b, const lumx_radiance_params* params, int reps, int* best_block,
+                        float* best_ms);
+/* Pins the batched radiance workgroup size: 256, 512 or 1024 threads (0
+ * restores the default, 256).  Results do not depend on it. */
+int lumx_batch_set_block(lumx_batch_t b, int block);
 /* Glossy pass of every world in one launch (one G-buffer and camera per world). */
 int lumx_batch_glossy(lumx_batch_t b, const uint8_t* const* matnorm_dev, const float* const* depth_dev,
                       const int* width, const int* height, const lumx_camera* cams, float max_dist, int max_steps,

@@ -309,6 +309,18 @@ class Batch:
         nr = (C.c_int * n)(*[0 if r is None else int(r.shape[0]) for r in requests])
         check(self.lib.lumx_batch_radiance(self.handle, rq, nr, ps, _stream()))
 
+    def autotune(self, params, reps=3):
+        """Choose the batched radiance workgroup size; returns (block, ms)."""
+        n = len(self.worlds)
+        ps = (RadianceParams * n)(*params)
+        blk = C.c_int()
+        ms = C.c_float()
+        check(self.lib.lumx_batch_autotune(self.handle, ps, reps, C.byref(blk), C.byref(ms)))
+        return blk.value, ms.value
+
+    def set_block(self, block):
+        check(self.lib.lumx_batch_set_block(self.handle, int(block)))
+
     def glossy(self, matnorms, depths, cams, outs, max_dist=128.0, max_steps=0):
         n = len(self.worlds)
         mn = (C.c_void_p * n)(*[m.data_ptr() for m in matnorms])

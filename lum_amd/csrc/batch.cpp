@@ -49,6 +49,7 @@ struct Launcher {
     std::vector<DescSlot> captured;      // owned by recorded graphs until destroy
     int ring_next = 0;
     int* frame_dev = nullptr;
+    int block = 0;  // radiance workgroup size chosen by the autotuner (0 = 256)
 };
 
 static void free_slot(DescSlot& s) {
@@ -144,7 +145,7 @@ int launcher_set_frame(Launcher* L, int value, int add, hipStream_t s) {
 
 int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32_t* const* requests,
                       const int* n_requests, const lumx_radiance_params* params, bool use_frame, hipStream_t s,
-                      bool served) {
+                      bool served, bool commit) {
     if (n <= 0) return LUMX_OK;
     if (n > L->max_worlds) return fail(LUMX_EINVAL, "%d worlds exceed the launcher's %d", n, L->max_worlds);
     const int format = worlds[0]->desc.radiance_format;
@@ -164,7 +165,7 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
         max_r = std::max(max_r, r);
     }
     HIP_TRY(hipSetDevice(L->ctx->device));
-    const int block = std::max(256, max_r);
+    const int block = std::max(L->block > 0 ? L->block : 256, max_r);
     DescSlot* slot = nullptr;
     int rc = acquire_slot(L, s, &slot);
     if (rc) return rc;
@@ -191,11 +192,14 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
     const RadianceArgs* args_dev = (const RadianceArgs*)slot->dev;
     HIP_TRY(launch_radiance_batch(args_dev, (const int*)((char*)slot->dev + args_bytes), n, (int)total, block, format,
                                   served, use_frame ? L->frame_dev : nullptr, s));
-    // Subset updates land in their current fields in place.
-    HIP_TRY(launch_scatter_batch(args_dev, (const int*)((char*)slot->dev + args_bytes + begin_bytes), n,
-                                 (int)total_staged, (int)worlds[0]->texel_bytes, s));
-    for (int i = 0; i < n; ++i)
-        if (!args[i].staged) worlds[i]->cur ^= 1;  // full updates wrote the other field
+    // Subset updates land in their current fields in place.  A trial launch
+    // (commit = false) writes the free field buffers only.
+    if (commit) {
+        HIP_TRY(launch_scatter_batch(args_dev, (const int*)((char*)slot->dev + args_bytes + begin_bytes), n,
+                                     (int)total_staged, (int)worlds[0]->texel_bytes, s));
+        for (int i = 0; i < n; ++i)
+            if (!args[i].staged) worlds[i]->cur ^= 1;  // full updates wrote the other field
+    }
     return release_slot(slot, s);
 }
 
@@ -291,7 +295,62 @@ int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests, const in
                         const lumx_radiance_params* params, void* stream) {
     if (!b || !params) return fail(LUMX_EINVAL, "null argument");
     return launcher_radiance(b->L, b->worlds.data(), (int)b->worlds.size(), requests, n_requests, params, true,
-                             (hipStream_t)stream, false);
+                             (hipStream_t)stream, false, true);
+}
+
+int lumx_batch_autotune(lumx_batch_t b, const lumx_radiance_params* params, int reps, int* best_block,
+                        float* best_ms) {
+    if (!b || !params) return fail(LUMX_EINVAL, "null argument");
+    if (reps < 1) reps = 1;
+    Launcher* L = b->L;
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    hipStream_t s = L->ctx->stream;
+    const int n = (int)b->worlds.size();
+    HIP_TRY(hipDeviceSynchronize());  // work queued on other streams must not overlap the timings
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int saved = L->block;
+    int best = saved, rc = LUMX_OK;
+    float best_t = 1e30f;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int cand : {256, 512, 1024}) {
+        if (e != hipSuccess || rc) break;
+        L->block = cand;
+        // Every probe of every world, written to the free field buffers only:
+        // the fields the worlds read from are left as they were.
+        rc = launcher_radiance(L, b->worlds.data(), n, nullptr, nullptr, params, true, s, false, false);
+        if (rc) break;
+        e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && rc == 0 && e == hipSuccess; ++r)
+            rc = launcher_radiance(L, b->worlds.data(), n, nullptr, nullptr, params, true, s, false, false);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && !rc && ms / (float)reps < best_t) {
+            best_t = ms / (float)reps;
+            best = cand;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    L->block = saved;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(LUMX_EDEVICE, "autotune: %s", hipGetErrorString(e));
+    L->block = best;
+    if (best_block) *best_block = best;
+    if (best_ms) *best_ms = best_t;
+    clear_error();
+    return LUMX_OK;
+}
+
+int lumx_batch_set_block(lumx_batch_t b, int block) {
+    if (!b) return fail(LUMX_EINVAL, "null batch");
+    if (block != 0 && block != 256 && block != 512 && block != 1024)
+        return fail(LUMX_EINVAL, "batched workgroup size %d is not 0, 256, 512 or 1024", block);
+    b->L->block = block;
+    clear_error();
+    return LUMX_OK;
 }
 
 int lumx_batch_glossy(lumx_batch_t b, const uint8_t* const* matnorm, const float* const* depth, const int* width,

@@ -113,7 +113,7 @@ void launcher_destroy(Launcher* L);
 int launcher_set_frame(Launcher* L, int value, int add, hipStream_t s);
 int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32_t* const* requests,
                       const int* n_requests, const lumx_radiance_params* params, bool use_frame, hipStream_t s,
-                      bool served);
+                      bool served, bool commit = true);
 int launcher_glossy(Launcher* L, lumx_world* const* worlds, int n, const uint8_t* const* matnorm,
                     const float* const* depth, const int* width, const int* height, const lumx_camera* cams,
                     const float* max_dist, const int* max_steps, float* const* out, hipStream_t s);

@@ -95,6 +95,44 @@ def test_batch_frame_counter_offsets_seed(gpu_ctx):
         s.close()
 
 
+def test_batch_autotune_and_block_sizes(gpu_ctx):
+    """The autotuner's trial launches leave every field as it was, and each
+    workgroup size it chooses from gives the single-world result (tail
+    workgroups of odd probe counts with 128 rays included)."""
+    import torch
+    batch_worlds, single_worlds = [], []
+    for size, rays, seed in SPECS:
+        w, _ = make_world(gpu_ctx, size, 40 + seed)
+        batch_worlds.append(w)
+        s, _ = make_world(gpu_ctx, size, 40 + seed)
+        s.set_tuning(variant=0)
+        single_worlds.append(s)
+    b = Batch(gpu_ctx, batch_worlds)
+    try:
+        params = [radiance_params(rays=r, time_seed=5 + i, alpha=0.3) for i, (_, r, _) in enumerate(SPECS)]
+        before = [w.read_radiance() for w in batch_worlds]
+        block, ms = b.autotune(params, reps=2)
+        assert block in (256, 512, 1024) and ms > 0
+        for w, f in zip(batch_worlds, before):
+            assert np.array_equal(w.read_radiance(), f)
+        for blk in (256, 512, 1024, 0):
+            b.set_block(blk)
+            b.radiance(params)
+            for s, p in zip(single_worlds, params):
+                s.radiance_update(None, p)
+            torch.cuda.synchronize()
+            for i, (bw, sw) in enumerate(zip(batch_worlds, single_worlds)):
+                assert np.array_equal(bw.read_radiance(), sw.read_radiance()), f"block {blk} world {i}"
+            for p in params:
+                p.time_seed += 1
+        with pytest.raises(Exception):
+            b.set_block(384)
+    finally:
+        b.close()
+        for w in batch_worlds + single_worlds:
+            w.close()
+
+
 def test_batch_glossy_equals_single(gpu_ctx):
     import torch
     worlds, datas = [], []
