@@ -104,14 +104,15 @@ def run_plain(args):
 
     if args.sweep:
         sweep = []
-        for variant, threads, P, eb, pb in ([(0, 0, 0, 0, 0), (2, 0, 0, 0, 0)] +
-                                            [(1, th, P, eb, pb) for th in (256, 512, 1024) for P in (4, 16, 64)
-                                             for eb in (0, 1) for pb in (0, 1)]):
+        for variant, threads, P, eb, pb, k in ([(0, 0, 0, 0, 0, 1), (0, 0, 0, 0, 0, 2), (0, 0, 0, 0, 0, 4),
+                                                 (2, 0, 0, 0, 0, 1)] +
+                                                [(1, th, P, eb, pb, 1) for th in (256, 512, 1024) for P in (4, 16, 64)
+                                                 for eb in (0, 1) for pb in (0, 1)]):
             if P * rays > 4096:
                 continue
             try:
                 w.set_tuning(variant=variant, threads=threads or 256, probes_per_batch=P or 16,
-                             lds_block_bits=eb, lds_palette_bits=pb, wg_per_cu=0)
+                             lds_block_bits=eb, lds_palette_bits=pb, wg_per_cu=0, rays_per_lane=k)
                 w.radiance_update(None, params(0))
                 torch.cuda.synchronize()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -120,10 +121,11 @@ def run_plain(args):
                     w.radiance_update(None, params(k))
                 e1.record()
                 torch.cuda.synchronize()
-                sweep.append(dict(variant=variant, threads=threads, P=P, eb=eb, pb=pb,
+                sweep.append(dict(variant=variant, threads=threads, P=P, eb=eb, pb=pb, rays_per_lane=k,
                                   ms=e0.elapsed_time(e1) / 10))
             except Exception as ex:  # shape does not fit
-                sweep.append(dict(variant=variant, threads=threads, P=P, eb=eb, pb=pb, error=str(ex)[:80]))
+                sweep.append(dict(variant=variant, threads=threads, P=P, eb=eb, pb=pb, rays_per_lane=k,
+                                  error=str(ex)[:80]))
         print(json.dumps({"sweep": sweep}), file=sys.stderr, flush=True)
         w.write_radiance(field)
     tuning, tune_ms = w.autotune(params(0), reps=5)
@@ -247,7 +249,7 @@ def run_batched(args):
 
     for c in cases:
         c.params.time_seed = 0
-    batch_block, _ = batch.autotune(params, reps=3)
+    batch_block, batch_rpl, _ = batch.autotune(params, reps=3)
     for k in range(args.warmup):
         plain_frame(k)
         batched_frame(k)
@@ -307,7 +309,7 @@ def run_batched(args):
         "plain_value": plain_value,
         "speedup_vs_plain": value / plain_value,
         "per_frame": per_frame,
-        "batch_block": batch_block,
+        "batch_shape": {"block": batch_block, "rays_per_lane": batch_rpl},
     }
     batch.close()
     for c in cases:
@@ -388,8 +390,22 @@ def run_serving(args):
     probe_rays = sum(cnt * rays for kind, _, _, cnt, rays in trace if kind == "r")
     pixels = sum(W * H for kind, *_ in trace if kind == "g")
 
+    # Launch shape of the served radiance launches (workgroup size x rays per
+    # lane), chosen by the batch autotuner over max_batch random resident
+    # worlds with rays from the request mix; the plain engine keeps the
+    # default shape (256 threads, one ray per thread).
+    pick = rng.choice(n_worlds, size=min(args.max_batch, n_worlds), replace=False)
+    tune_batch = Batch(ctx, [worlds[i] for i in pick])
+    tune_params = [radiance_params(rays=int(rng.choice([32, 64, 128])), time_seed=0, alpha=0.1, max_steps=64)
+                   for _ in pick]
+    served_block, served_rpl, _ = tune_batch.autotune(tune_params, reps=3)
+    tune_batch.close()
+    torch.cuda.synchronize()
+
     def serve(max_batch, streams, arrivals):
-        srv = Server(ctx, worlds, hbm_budget=args.hbm_budget_gb * (1 << 30), max_batch=max_batch, streams=streams)
+        shape = (served_block, served_rpl) if max_batch > 1 else (0, 0)
+        srv = Server(ctx, worlds, hbm_budget=args.hbm_budget_gb * (1 << 30), max_batch=max_batch, streams=streams,
+                     block=shape[0], rays_per_lane=shape[1])
         lat, wall = srv.run_trace(build(srv), arrivals)
         st = srv.stats()
         srv.close()
@@ -417,6 +433,8 @@ def run_serving(args):
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     batch = Batch(ctx, worlds)
+    batch.set_shape(served_block, served_rpl)
+    kernel_name = "radiance_batch_kernel<1,false,false,%d>" % served_rpl
     rp = [radiance_params(rays=64, time_seed=0, alpha=0.1, max_steps=64) for _ in worlds]
     bytes_per_launch = 0
     for w in worlds:
@@ -453,6 +471,7 @@ def run_serving(args):
                                "%d requests, 80%% radiance (1-100%% probe subset, 32/64/128 rays, max 64 steps), "
                                "20%% glossy 640x360 (max 128 steps)" % (n_worlds, n_req),
                    "max_batch": args.max_batch, "streams": args.streams,
+                   "served_shape": {"block": served_block, "rays_per_lane": served_rpl},
                    "hbm_budget_gb": args.hbm_budget_gb, "probe_rays": probe_rays, "glossy_pixels": pixels},
         "plain_value": plain_value,
         "speedup_vs_plain": value / plain_value,
@@ -465,8 +484,8 @@ def run_serving(args):
                       "plain_p50_ms": pct(plain_lat_s, 50), "served_p50_ms": pct(served_lat_s, 50)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("radiance_batch_kernel<1,false,false>"),
-                     "kernel": "radiance_batch_kernel<1,false,false>", "kernel_ms": kern_ms,
+                     "traffic": measured_traffic(kernel_name),
+                     "kernel": kernel_name, "kernel_ms": kern_ms,
                      "bytes_per_launch": bytes_per_launch,
                      "launch": "all %d worlds, every probe, 64 rays, FP16" % n_worlds},
     }

@@ -135,6 +135,9 @@ typedef struct {
     int lds_block_bits;     /* stage the empty-block bits in LDS when they fit */
     int lds_palette_bits;   /* stage the palette occupancy bits in LDS when they fit */
     int wg_per_cu;          /* persistent workgroups per CU, 0 = occupancy limit */
+    int rays_per_lane;      /* variant 0: rays traced one after another by one thread (0 or 1, 2, 4);
+                               used when rays per probe is a power of two >= it, else 1.  The
+                               results do not depend on it. */
 } lumx_radiance_tuning;
 
 /* ---- context / errors ---- */
@@ -285,10 +288,11 @@ int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests_dev, cons
  * (the worlds' fields are left unchanged), keeps the fastest for this
  * batch's later launches and returns it.  No reference counterpart. */
 int lumx_batch_autotune(lumx_batch_t b, const lumx_radiance_params* params, int reps, int* best_block,
-                        float* best_ms);
-/* Pins the batched radiance workgroup size: 256, 512 or 1024 threads (0
- * restores the default, 256).  Results do not depend on it. */
-int lumx_batch_set_block(lumx_batch_t b, int block);
+                        int* best_rays_per_lane, float* best_ms);
+/* Pins the batched radiance launch shape: workgroup size 256, 512 or 1024
+ * (0 = 256) and rays per lane 1, 2 or 4 (0 = 1; a launch whose smallest
+ * rays per probe is below it uses 1).  Results do not depend on either. */
+int lumx_batch_set_shape(lumx_batch_t b, int block, int rays_per_lane);
 /* Glossy pass of every world in one launch (one G-buffer and camera per world). */
 int lumx_batch_glossy(lumx_batch_t b, const uint8_t* const* matnorm_dev, const float* const* depth_dev,
                       const int* width, const int* height, const lumx_camera* cams, float max_dist, int max_steps,
@@ -317,7 +321,8 @@ typedef struct {
     size_t hbm_budget;  /* bytes of resident world state, 0 = no limit */
     int max_batch;      /* requests per launch; 1 = one request per launch */
     int streams;        /* launches in flight at once (>= 1) */
-    int reserved;
+    int block;          /* radiance workgroup size: 0 (= 256), 256, 512 or 1024 */
+    int rays_per_lane;  /* rays per thread of the radiance launches: 0 (= 1), 1, 2 or 4 */
 } lumx_server_desc;
 
 typedef struct {

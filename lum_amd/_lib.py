@@ -50,7 +50,8 @@ class Placement(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("variant", C.c_int), ("threads", C.c_int), ("probes_per_batch", C.c_int),
-                ("lds_block_bits", C.c_int), ("lds_palette_bits", C.c_int), ("wg_per_cu", C.c_int)]
+                ("lds_block_bits", C.c_int), ("lds_palette_bits", C.c_int), ("wg_per_cu", C.c_int),
+                ("rays_per_lane", C.c_int)]
 
 
 class Camera(C.Structure):
@@ -63,7 +64,8 @@ class Lightmap(C.Structure):
 
 
 class ServerDesc(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("streams", C.c_int), ("reserved", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("streams", C.c_int), ("block", C.c_int),
+                ("rays_per_lane", C.c_int)]
 
 
 class Request(C.Structure):
@@ -120,8 +122,9 @@ _SIGS = {
     "lumx_radiance_update_counted": (_I, [_P, _P, _P, _I, C.POINTER(RadianceParams), _P]),
     "lumx_world_shift_radiance": (_I, [_P, _I, _I, _I, _P]),
     "lumx_glossy": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.c_float, _I, _P, _P, _P]),
-    "lumx_batch_autotune": (_I, [_P, C.POINTER(RadianceParams), _I, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
-    "lumx_batch_set_block": (_I, [_P, _I]),
+    "lumx_batch_autotune": (_I, [_P, C.POINTER(RadianceParams), _I, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_float)]),
+    "lumx_batch_set_shape": (_I, [_P, _I, _I]),
     "lumx_map_models_device": (_I, [_P, _I, _P, _P, _P]),
     "lumx_vox_parse": (_I, [_P, C.c_size_t, _P, _P, C.c_size_t, _P, _P]),
     "lumx_diffuse": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.POINTER(Lightmap), _P, _P]),

@@ -12,7 +12,8 @@ from . import _lib
 from ._lib import (Camera, Placement, RadianceParams, Transform, Tuning, WorldDesc, check, load, RADIANCE_F32,
                    PALETTE_XYZ_XFAST, ServerDesc, Request, ServerStats, REQ_RADIANCE, REQ_GLOSSY)
 
-TUNING_FIELDS = ("variant", "threads", "probes_per_batch", "lds_block_bits", "lds_palette_bits", "wg_per_cu")
+TUNING_FIELDS = ("variant", "threads", "probes_per_batch", "lds_block_bits", "lds_palette_bits", "wg_per_cu",
+                 "rays_per_lane")
 
 
 def _stream():
@@ -310,16 +311,17 @@ class Batch:
         check(self.lib.lumx_batch_radiance(self.handle, rq, nr, ps, _stream()))
 
     def autotune(self, params, reps=3):
-        """Choose the batched radiance workgroup size; returns (block, ms)."""
+        """Choose the batched radiance launch shape; returns (block, rays_per_lane, ms)."""
         n = len(self.worlds)
         ps = (RadianceParams * n)(*params)
         blk = C.c_int()
+        rpl = C.c_int()
         ms = C.c_float()
-        check(self.lib.lumx_batch_autotune(self.handle, ps, reps, C.byref(blk), C.byref(ms)))
-        return blk.value, ms.value
+        check(self.lib.lumx_batch_autotune(self.handle, ps, reps, C.byref(blk), C.byref(rpl), C.byref(ms)))
+        return blk.value, rpl.value, ms.value
 
-    def set_block(self, block):
-        check(self.lib.lumx_batch_set_block(self.handle, int(block)))
+    def set_shape(self, block=0, rays_per_lane=0):
+        check(self.lib.lumx_batch_set_shape(self.handle, int(block), int(rays_per_lane)))
 
     def glossy(self, matnorms, depths, cams, outs, max_dist=128.0, max_steps=0):
         n = len(self.worlds)
@@ -392,9 +394,9 @@ def glossy_request(world_id, matnorm, depth, cam, out, max_dist=128.0, max_steps
 class Server:
     """Continuous-batching request engine (lumx_server_*)."""
 
-    def __init__(self, ctx, worlds, hbm_budget=0, max_batch=32, streams=2):
+    def __init__(self, ctx, worlds, hbm_budget=0, max_batch=32, streams=2, block=0, rays_per_lane=0):
         self.lib = ctx.lib
-        d = ServerDesc(int(hbm_budget), int(max_batch), int(streams), 0)
+        d = ServerDesc(int(hbm_budget), int(max_batch), int(streams), int(block), int(rays_per_lane))
         self.handle = C.c_void_p()
         check(self.lib.lumx_server_create(ctx.handle, C.byref(d), C.byref(self.handle)))
         self.ids = []

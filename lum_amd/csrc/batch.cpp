@@ -49,8 +49,19 @@ struct Launcher {
     std::vector<DescSlot> captured;      // owned by recorded graphs until destroy
     int ring_next = 0;
     int* frame_dev = nullptr;
-    int block = 0;  // radiance workgroup size chosen by the autotuner (0 = 256)
+    int block = 0;          // radiance workgroup size (0 = 256)
+    int rays_per_lane = 0;  // rays per thread of the radiance launches (0 = 1)
 };
+
+int launcher_set_shape(Launcher* L, int block, int rays_per_lane) {
+    if (block != 0 && block != 256 && block != 512 && block != 1024)
+        return fail(LUMX_EINVAL, "batched workgroup size %d is not 0, 256, 512 or 1024", block);
+    if (rays_per_lane < 0 || rays_per_lane > 4 || rays_per_lane == 3)
+        return fail(LUMX_EINVAL, "rays per lane %d is not 0, 1, 2 or 4", rays_per_lane);
+    L->block = block;
+    L->rays_per_lane = rays_per_lane;
+    return LUMX_OK;
+}
 
 static void free_slot(DescSlot& s) {
     if (s.done) hipEventDestroy(s.done);
@@ -149,7 +160,7 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
     if (n <= 0) return LUMX_OK;
     if (n > L->max_worlds) return fail(LUMX_EINVAL, "%d worlds exceed the launcher's %d", n, L->max_worlds);
     const int format = worlds[0]->desc.radiance_format;
-    int max_r = 1;
+    int max_r = 1, min_r = 1 << 30;
     for (int i = 0; i < n; ++i) {
         const int32_t* req = requests ? requests[i] : nullptr;
         if (req && !n_requests) return fail(LUMX_EINVAL, "request lists need their counts");
@@ -163,9 +174,12 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
         for (int j = 0; j < i; ++j)
             if (worlds[j] == worlds[i]) return fail(LUMX_EINVAL, "world %d appears twice in one launch", i);
         max_r = std::max(max_r, r);
+        min_r = std::min(min_r, r);
     }
     HIP_TRY(hipSetDevice(L->ctx->device));
-    const int block = std::max(L->block > 0 ? L->block : 256, max_r);
+    // K rays per thread: a probe of R rays owns R / K threads of its world's range.
+    const int K = rays_per_lane_for(min_r, L->rays_per_lane);
+    const int block = std::max(L->block > 0 ? L->block : 256, max_r / K);
     DescSlot* slot = nullptr;
     int rc = acquire_slot(L, s, &slot);
     if (rc) return rc;
@@ -182,7 +196,7 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
         rc = radiance_args(w, req, nr, &params[i], &args[i]);
         if (rc) return rc;
         begin[i] = (int)total;
-        total += round_up((size_t)args[i].n_probes * args[i].rays, (size_t)block);
+        total += round_up((size_t)args[i].n_probes * (args[i].rays / K), (size_t)block);
         if (total > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batch too large");
         pbegin[i] = (int)total_staged;
         if (args[i].staged) total_staged += args[i].n_probes;
@@ -190,8 +204,8 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
     pbegin[n] = (int)total_staged;
     HIP_TRY(hipMemcpyAsync(slot->dev, slot->host, args_bytes + 2 * begin_bytes, hipMemcpyHostToDevice, s));
     const RadianceArgs* args_dev = (const RadianceArgs*)slot->dev;
-    HIP_TRY(launch_radiance_batch(args_dev, (const int*)((char*)slot->dev + args_bytes), n, (int)total, block, format,
-                                  served, use_frame ? L->frame_dev : nullptr, s));
+    HIP_TRY(launch_radiance_batch(args_dev, (const int*)((char*)slot->dev + args_bytes), n, (int)total, block, K,
+                                  format, served, use_frame ? L->frame_dev : nullptr, s));
     // Subset updates land in their current fields in place.  A trial launch
     // (commit = false) writes the free field buffers only.
     if (commit) {
@@ -299,56 +313,65 @@ int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests, const in
 }
 
 int lumx_batch_autotune(lumx_batch_t b, const lumx_radiance_params* params, int reps, int* best_block,
-                        float* best_ms) {
+                        int* best_rays_per_lane, float* best_ms) {
     if (!b || !params) return fail(LUMX_EINVAL, "null argument");
     if (reps < 1) reps = 1;
     Launcher* L = b->L;
     HIP_TRY(hipSetDevice(L->ctx->device));
     hipStream_t s = L->ctx->stream;
     const int n = (int)b->worlds.size();
+    int min_r = 1 << 30;
+    for (int i = 0; i < n; ++i) min_r = std::min(min_r, params[i].rays);
     HIP_TRY(hipDeviceSynchronize());  // work queued on other streams must not overlap the timings
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const int saved = L->block;
-    int best = saved, rc = LUMX_OK;
+    const int saved_block = L->block, saved_k = L->rays_per_lane;
+    int best_blk = saved_block, best_k = saved_k, rc = LUMX_OK;
     float best_t = 1e30f;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    for (int cand : {256, 512, 1024}) {
-        if (e != hipSuccess || rc) break;
-        L->block = cand;
-        // Every probe of every world, written to the free field buffers only:
-        // the fields the worlds read from are left as they were.
-        rc = launcher_radiance(L, b->worlds.data(), n, nullptr, nullptr, params, true, s, false, false);
-        if (rc) break;
-        e = hipEventRecord(e0, s);
-        for (int r = 0; r < reps && rc == 0 && e == hipSuccess; ++r)
+    for (int k : {1, 2, 4}) {
+        if (k > 1 && rays_per_lane_for(min_r, k) != k) continue;
+        for (int cand : {256, 512, 1024}) {
+            if (e != hipSuccess || rc) break;
+            L->block = cand;
+            L->rays_per_lane = k;
+            // Every probe of every world, written to the free field buffers
+            // only: the fields the worlds read from are left as they were.
             rc = launcher_radiance(L, b->worlds.data(), n, nullptr, nullptr, params, true, s, false, false);
-        if (e == hipSuccess) e = hipEventRecord(e1, s);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0.0f;
-        if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e == hipSuccess && !rc && ms / (float)reps < best_t) {
-            best_t = ms / (float)reps;
-            best = cand;
+            if (rc) break;
+            e = hipEventRecord(e0, s);
+            for (int r = 0; r < reps && rc == 0 && e == hipSuccess; ++r)
+                rc = launcher_radiance(L, b->worlds.data(), n, nullptr, nullptr, params, true, s, false, false);
+            if (e == hipSuccess) e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && !rc && ms / (float)reps < best_t) {
+                best_t = ms / (float)reps;
+                best_blk = cand;
+                best_k = k;
+            }
         }
     }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
-    L->block = saved;
+    L->block = saved_block;
+    L->rays_per_lane = saved_k;
     if (rc) return rc;
     if (e != hipSuccess) return fail(LUMX_EDEVICE, "autotune: %s", hipGetErrorString(e));
-    L->block = best;
-    if (best_block) *best_block = best;
+    L->block = best_blk;
+    L->rays_per_lane = best_k;
+    if (best_block) *best_block = best_blk;
+    if (best_rays_per_lane) *best_rays_per_lane = best_k;
     if (best_ms) *best_ms = best_t;
     clear_error();
     return LUMX_OK;
 }
 
-int lumx_batch_set_block(lumx_batch_t b, int block) {
+int lumx_batch_set_shape(lumx_batch_t b, int block, int rays_per_lane) {
     if (!b) return fail(LUMX_EINVAL, "null batch");
-    if (block != 0 && block != 256 && block != 512 && block != 1024)
-        return fail(LUMX_EINVAL, "batched workgroup size %d is not 0, 256, 512 or 1024", block);
-    b->L->block = block;
+    int rc = launcher_set_shape(b->L, block, rays_per_lane);
+    if (rc) return rc;
     clear_error();
     return LUMX_OK;
 }

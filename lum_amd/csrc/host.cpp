@@ -609,12 +609,15 @@ static int commit_blocks(lumx_world* w, hipStream_t s) {
     int16_t* stage_blocks = w->stage_blocks;
     int32_t* stage_copy = w->stage_copy_src;
     if (stream_capturing(s)) {
+        // Blocks, then (16-byte aligned) the copy list and the counter word:
+        // the same layout as the eager staging's palette_slots + 4 words.
         void* p = nullptr;
-        const size_t bytes = (size_t)w->n_blocks * 2 + (size_t)w->desc.palette_slots * 4 + 16;
+        const size_t copy_off = (((size_t)w->n_blocks * 2 + 15) / 16) * 16;
+        const size_t bytes = copy_off + ((size_t)w->desc.palette_slots + 4) * 4;
         HIP_TRY(hipHostMalloc(&p, bytes, hipHostMallocDefault));
         w->graph_stage.push_back(p);
         stage_blocks = (int16_t*)p;
-        stage_copy = (int32_t*)((char*)p + (((size_t)w->n_blocks * 2 + 15) / 16) * 16);
+        stage_copy = (int32_t*)((char*)p + copy_off);
     } else {
         if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
         w->stage_pending = false;
@@ -1131,6 +1134,8 @@ int lumx::radiance_check(const lumx_world* w, const int32_t* req, int n_req, con
 
 static int check_tuning(const lumx_world* w, const lumx_radiance_tuning& t, int rays) {
     if (t.variant < 0 || t.variant > 2) return fail(LUMX_EINVAL, "tuning variant %d invalid", t.variant);
+    if (t.rays_per_lane != 0 && t.rays_per_lane != 1 && t.rays_per_lane != 2 && t.rays_per_lane != 4)
+        return fail(LUMX_EINVAL, "tuning rays_per_lane %d is not 1, 2 or 4", t.rays_per_lane);
     if (t.variant != 1) return LUMX_OK;
     if (t.threads < 64 || t.threads > 1024 || t.threads % 64)
         return fail(LUMX_EINVAL, "tuning threads %d invalid", t.threads);
@@ -1195,7 +1200,7 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
     a.ray_keys = keys;
     a.ray_steps = steps;
     if (tune.variant == 0 || tune.variant == 2) {
-        HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, tune.variant == 2, s));
+        HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, tune.variant == 2, tune.rays_per_lane, s));
         if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
         return LUMX_OK;
     }
@@ -1266,6 +1271,12 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
     lumx_radiance_tuning c0{};
     c0.variant = 0;
     cands.push_back(c0);
+    for (int k : {2, 4})
+        if (rays_per_lane_for(p->rays, k) == k) {
+            c0.rays_per_lane = k;
+            cands.push_back(c0);
+        }
+    c0.rays_per_lane = 0;
     c0.variant = 2;
     cands.push_back(c0);
     const int threads[] = {256, 512, 1024};

@@ -110,6 +110,8 @@ bool stream_capturing(hipStream_t s);
 struct Launcher;
 int launcher_create(lumx_context* ctx, int max_worlds, int ring_slots, Launcher** out);
 void launcher_destroy(Launcher* L);
+// Radiance launch shape: workgroup size 0 (= 256), 256, 512, 1024; rays per lane 0 (= 1), 1, 2, 4.
+int launcher_set_shape(Launcher* L, int block, int rays_per_lane);
 int launcher_set_frame(Launcher* L, int value, int add, hipStream_t s);
 int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32_t* const* requests,
                       const int* n_requests, const lumx_radiance_params* params, bool use_frame, hipStream_t s,

@@ -157,7 +157,11 @@ int select_claim_idle();
 hipError_t launch_select(const SelectArgs& a, hipStream_t stream);
 hipError_t launch_shift(const ShiftArgs& a, int texel_bytes, hipStream_t stream);
 
-hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, hipStream_t stream);
+// rays_per_lane 2 or 4 (plain march, not traced) when rays is a power of
+// two >= it, else 1; rays_per_lane_for gives the value a launch uses.
+int rays_per_lane_for(int rays, int want);
+hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, int rays_per_lane,
+                           hipStream_t stream);
 // Applies a staged subset update: field_in[probe of i] = field_out[i].
 hipError_t launch_scatter(const RadianceArgs& a, int texel_bytes, hipStream_t stream);
 hipError_t launch_scatter_batch(const RadianceArgs* args_dev, const int* pbegin_dev, int n_worlds, int total_probes,
@@ -174,8 +178,8 @@ hipError_t launch_exec_copies(const CopyArgs& a, hipStream_t stream);
 hipError_t launch_glossy(const GlossyArgs& a, int store, bool traced, hipStream_t stream);
 hipError_t launch_diffuse(const DiffuseArgs& a, int store, hipStream_t stream);
 hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
-                                 int total_threads, int block, int store, bool served, const int* frame_dev,
-                                 hipStream_t stream);
+                                 int total_threads, int block, int rays_per_lane, int store, bool served,
+                                 const int* frame_dev, hipStream_t stream);
 hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_threads, int store,
                                hipStream_t stream);
 

@@ -58,120 +58,210 @@ __device__ __forceinline__ int dispatched_probes(const RadianceArgs& a) {
     return min(a.n_probes, gld(a.n_requests_dev, 0) / a.size);
 }
 
-// One ray per thread; tid is the thread's index within this world's launch
-// range, which starts on a workgroup boundary.
-template <int STORE, bool TRACED, bool OCC>
-__device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const int tid, f3* chunk_sum) {
-    const int R = a.rays;
-    int wg = a.log2_rays >= 0 ? (tid >> a.log2_rays) : tid / R;
-    // With R >= 64 a wave holds rays of one probe: keep the probe on the scalar unit.
-    if (R >= 64) wg = __builtin_amdgcn_readfirstlane(wg);
-    const int lane = tid - wg * R;
-    const bool active = wg < dispatched_probes(a);
-
+// Probe wg's block coordinates (from the request list or the x-fastest
+// index); false when outside the world or not dispatched.
+__device__ __forceinline__ bool probe_coords(const RadianceArgs& a, int wg, int* px, int* py, int* pz) {
+    if (wg >= dispatched_probes(a)) return false;
+    if (a.requests) {
+        const int4 q = gld(a.requests, (size_t)(wg * a.size + a.shift));
+        *px = q.x;
+        *py = q.y;
+        *pz = q.z;
+    } else {
+        probe_xyz(a, (uint32_t)wg, px, py, pz);
+    }
     const WorldView& w = a.world;
-    int px = 0, py = 0, pz = 0;
-    bool valid = false;
-    f3 light = {0.0f, 0.0f, 0.0f};
-    if (active) {
-        if (a.requests) {
-            int4 q = gld(a.requests, (size_t)(wg * a.size + a.shift));
-            px = q.x;
-            py = q.y;
-            pz = q.z;
-        } else {
-            probe_xyz(a, (uint32_t)wg, &px, &py, &pz);
-        }
-        valid = (unsigned)px < (unsigned)w.wx && (unsigned)py < (unsigned)w.wy && (unsigned)pz < (unsigned)w.wz;
-    }
-    if (valid) {
-        const uint32_t t = (uint32_t)a.time_seed;
-        const uint32_t gidx = (uint32_t)wg * (uint32_t)R + (uint32_t)lane;
-        f3 r0, r1;
-        hash33(gidx + t, t, t, &r0);
-        hash33(gidx + t + 1u, t, t, &r1);
-        const float4 fb = gld(a.fib, (size_t)lane);
-        f3 corner = f3{(float)px, (float)py, (float)pz} * 16.0f;
-        f3 center = corner + f3{8.0f, 8.0f, 8.0f};
-        const f3 half = {0.5f, 0.5f, 0.5f};
-        f3 dir = normalize3(f3{fb.x, fb.y, fb.z} + 0.1f * ((r0 - half) * 2.0f));
-        f3 pos = center + dir * 6.0f;
-        pos = pos + (r1 - half) * 2.0f;
-        if (lane & 1) dir = -dir;
-
-        RayHit h;
-        if (OCC) {
-            OccView occ;
-            occ.ids = w.blocks;
-            occ.palette = w.palette;
-            occ.eb = a.empty_bits;
-            occ.pb = a.pal_bits;
-            occ.wx = w.wx;
-            occ.wy = w.wy;
-            occ.wz = w.wz;
-            occ.palette_slots = w.palette_slots;
-            occ.block0_empty = w.block0_empty;
-            h = cast_ray_occ(occ, pos, dir, a.max_dist, a.max_steps);
-        } else {
-            h = cast_ray(w, pos, dir, a.max_dist, a.max_steps);
-        }
-        if (h.hit) {
-            MatF m = get_mat(w, h.voxel);
-            f3 prop = sample_radiance<STORE>(a.field_in, w.wx, w.wy, w.wz, pos + h.fraction * dir, h.normal);
-            light = light + (prop * m.color + m.color * m.emit);
-        } else {
-            float gl = -dot3(dir, radiance_light_dir());
-            if (gl > 0.95f) {
-                f3 sun = f3{0.9f, 0.9f, 0.6f} * 3.0f;
-                light = light + gl * sun;
-            }
-            f3 sky = f3{0.53f, 0.81f, 0.92f} * 0.1f;
-            light = light + sky;
-        }
-        if (TRACED) {
-            size_t ri = (size_t)wg * R + lane;
-            if (a.ray_keys) gst(a.ray_keys, ri, h.hit ? h.key : 0);
-            if (a.ray_steps) {
-                gst(a.ray_steps, 2 * ri + 0, h.coarse);
-                gst(a.ray_steps, 2 * ri + 1, h.refine);
-            }
-        }
-    }
-
-    // Butterfly over C = min(R, 64) lanes: v[i] += v[i ^ s], s = C/2 .. 1.
-    const int C = R < 64 ? R : 64;
-    for (int s = C >> 1; s >= 1; s >>= 1) {
-        float ox = __shfl_xor(light.x, s);
-        float oy = __shfl_xor(light.y, s);
-        float oz = __shfl_xor(light.z, s);
-        light = light + f3{ox, oy, oz};
-    }
-    f3 total = light;
-    if (R > 64) {
-        const int wave = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) chunk_sum[wave] = light;
-        __syncthreads();
-        if (lane == 0) {
-            const int n_chunks = R >> 6;
-            total = chunk_sum[wave];
-            for (int c = 1; c < n_chunks; ++c) total = total + chunk_sum[wave + c];
-        }
-    }
-    if (valid && lane == 0) {
-        f3 mean = probe_mean(total, R);
-        const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
-        f3 old = Texel<STORE>::load(a.field_in, ti);
-        old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
-        const float al = a.alpha, bl = 1.0f - a.alpha;
-        f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
-        Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
-    }
+    return (unsigned)*px < (unsigned)w.wx && (unsigned)*py < (unsigned)w.wy && (unsigned)*pz < (unsigned)w.wz;
 }
 
+// Light gathered by ray r of probe wg at block (px, py, pz).
 template <int STORE, bool TRACED, bool OCC>
+__device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int r, int px, int py, int pz) {
+    const WorldView& w = a.world;
+    const int R = a.rays;
+    const uint32_t t = (uint32_t)a.time_seed;
+    const uint32_t gidx = (uint32_t)wg * (uint32_t)R + (uint32_t)r;
+    f3 r0, r1;
+    hash33(gidx + t, t, t, &r0);
+    hash33(gidx + t + 1u, t, t, &r1);
+    const float4 fb = gld(a.fib, (size_t)r);
+    f3 corner = f3{(float)px, (float)py, (float)pz} * 16.0f;
+    f3 center = corner + f3{8.0f, 8.0f, 8.0f};
+    const f3 half = {0.5f, 0.5f, 0.5f};
+    f3 dir = normalize3(f3{fb.x, fb.y, fb.z} + 0.1f * ((r0 - half) * 2.0f));
+    f3 pos = center + dir * 6.0f;
+    pos = pos + (r1 - half) * 2.0f;
+    if (r & 1) dir = -dir;
+
+    RayHit h;
+    if (OCC) {
+        OccView occ;
+        occ.ids = w.blocks;
+        occ.palette = w.palette;
+        occ.eb = a.empty_bits;
+        occ.pb = a.pal_bits;
+        occ.wx = w.wx;
+        occ.wy = w.wy;
+        occ.wz = w.wz;
+        occ.palette_slots = w.palette_slots;
+        occ.block0_empty = w.block0_empty;
+        h = cast_ray_occ(occ, pos, dir, a.max_dist, a.max_steps);
+    } else {
+        h = cast_ray(w, pos, dir, a.max_dist, a.max_steps);
+    }
+    f3 light = {0.0f, 0.0f, 0.0f};
+    if (h.hit) {
+        MatF m = get_mat(w, h.voxel);
+        f3 prop = sample_radiance<STORE>(a.field_in, w.wx, w.wy, w.wz, pos + h.fraction * dir, h.normal);
+        light = light + (prop * m.color + m.color * m.emit);
+    } else {
+        float gl = -dot3(dir, radiance_light_dir());
+        if (gl > 0.95f) {
+            f3 sun = f3{0.9f, 0.9f, 0.6f} * 3.0f;
+            light = light + gl * sun;
+        }
+        f3 sky = f3{0.53f, 0.81f, 0.92f} * 0.1f;
+        light = light + sky;
+    }
+    if (TRACED) {
+        size_t ri = (size_t)wg * R + r;
+        if (a.ray_keys) gst(a.ray_keys, ri, h.hit ? h.key : 0);
+        if (a.ray_steps) {
+            gst(a.ray_steps, 2 * ri + 0, h.coarse);
+            gst(a.ray_steps, 2 * ri + 1, h.refine);
+        }
+    }
+    return light;
+}
+
+// v[i] += v[i ^ s] for s = width/2 .. 1 (width a power of two <= 64).
+__device__ __forceinline__ f3 butterfly(f3 v, int width) {
+    for (int s = width >> 1; s >= 1; s >>= 1) {
+        float ox = __shfl_xor(v.x, s);
+        float oy = __shfl_xor(v.y, s);
+        float oz = __shfl_xor(v.z, s);
+        v = v + f3{ox, oy, oz};
+    }
+    return v;
+}
+
+// Blend of the probe mean into the old texel (radiance.comp:349-354, 405-412).
+template <int STORE>
+__device__ __forceinline__ void store_probe(const RadianceArgs& a, int wg, int px, int py, int pz, f3 total) {
+    const WorldView& w = a.world;
+    f3 mean = probe_mean(total, a.rays);
+    const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
+    f3 old = Texel<STORE>::load(a.field_in, ti);
+    old = {fminf(fmaxf(old.x, 0.0f), 1.0f), fminf(fmaxf(old.y, 0.0f), 1.0f), fminf(fmaxf(old.z, 0.0f), 1.0f)};
+    const float al = a.alpha, bl = 1.0f - a.alpha;
+    f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
+    Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
+}
+
+// K rays per lane.  tid is the thread's index within this world's launch
+// range, which starts on a workgroup boundary; a probe owns L = R / K
+// consecutive threads.
+//
+// The probe total is reduced in one fixed order for every K: per chunk of
+// C = min(R, 64) rays a butterfly (v[i] += v[i ^ s], s = C/2 .. 1), then the
+// chunk sums in chunk order.  With K > 1 (R a power of two, R >= K) a lane
+// traces several rays one after another and adds them as the butterfly's
+// first stages would, so the total is bitwise the K = 1 total:
+//   Kc = min(K, R / C) chunks per lane, Kw = K / Kc rays per lane and chunk,
+//   Lc = C / Kw lanes per chunk.  Lane m = g * Lc + p traces, for each of its
+//   chunks c = g * Kc + jc, the rays c * C + p + o * Lc, o = 0, 1 (Kw = 2)
+//   or o = 0, 2, 1, 3 (Kw = 4): stage C/2 pairs rays p and p + C/2 (o = 0, 2
+//   for Kw = 4), stage C/4 pairs their sums.
+template <int STORE, bool TRACED, bool OCC, int K>
+__device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const int tid, f3* chunk_sum) {
+    const int R = a.rays;
+    constexpr int LOG2K = K == 1 ? 0 : (K == 2 ? 1 : 2);
+    const int L = R >> LOG2K;  // K > 1: R is a power of two
+    int wg = K == 1 ? (a.log2_rays >= 0 ? (tid >> a.log2_rays) : tid / R) : (tid >> (a.log2_rays - LOG2K));
+    // With L >= 64 a wave holds lanes of one probe: keep the probe on the scalar unit.
+    if (L >= 64) wg = __builtin_amdgcn_readfirstlane(wg);
+    const int m = tid - wg * L;
+    int px = 0, py = 0, pz = 0;
+    const bool valid = probe_coords(a, wg, &px, &py, &pz);
+    const f3 zero = {0.0f, 0.0f, 0.0f};
+    const int C = R < 64 ? R : 64;
+    f3 total;
+    if (K == 1) {
+        f3 light = valid ? probe_ray_light<STORE, TRACED, OCC>(a, wg, m, px, py, pz) : zero;
+        total = butterfly(light, C);
+        if (R > 64) {
+            const int wave = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) chunk_sum[wave] = total;
+            __syncthreads();
+            if (m == 0) {
+                const int n_chunks = R >> 6;
+                total = chunk_sum[wave];
+                for (int c = 1; c < n_chunks; ++c) total = total + chunk_sum[wave + c];
+            }
+        }
+    } else {
+        const int n_chunks = R / C;
+        const int Kc = K < n_chunks ? K : n_chunks;
+        const int Kw = K / Kc;
+        const int Lc = C / Kw;
+        const int g = m / Lc;
+        const int p = m - g * Lc;
+        f3 acc[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) acc[q] = zero;
+        f3 t0 = zero, t1 = zero, cur = zero;
+#pragma unroll 1
+        for (int j = 0; j < K; ++j) {
+            const int jc = j / Kw, jj = j - jc * Kw;
+            const int o = Kw == 4 ? (((jj & 1) << 1) | (jj >> 1)) : jj;
+            const int r = (g * Kc + jc) * C + p + o * Lc;
+            const f3 v = valid ? probe_ray_light<STORE, TRACED, OCC>(a, wg, r, px, py, pz) : zero;
+            if (Kw == 1) {
+                cur = v;
+            } else if (Kw == 2) {
+                if (jj == 0) t0 = v;
+                else cur = t0 + v;
+            } else {
+                if (jj == 0) t0 = v;
+                else if (jj == 1) t0 = t0 + v;
+                else if (jj == 2) t1 = v;
+                else cur = t0 + (t1 + v);
+            }
+            if (jj == Kw - 1) {
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    if (q == jc) acc[q] = cur;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+            if (q < Kc) acc[q] = butterfly(acc[q], Lc);
+        if (n_chunks == Kc) {
+            total = acc[0];
+#pragma unroll
+            for (int q = 1; q < K; ++q)
+                if (q < Kc) total = total + acc[q];
+        } else {
+            // Kc = K, Lc = 64: wave g of the probe holds chunks g*K .. g*K+K-1.
+            const int wave = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+                for (int q = 0; q < K; ++q) chunk_sum[wave * K + q] = acc[q];
+            }
+            __syncthreads();
+            total = acc[0];
+            if (m == 0) {
+                for (int c = 1; c < n_chunks; ++c) total = total + chunk_sum[wave * K + c];
+            }
+        }
+    }
+    if (valid && m == 0) store_probe<STORE>(a, wg, px, py, pz, total);
+}
+
+template <int STORE, bool TRACED, bool OCC, int K>
 __global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
     extern __shared__ f3 chunk_sum[];
-    radiance_lane_body<STORE, TRACED, OCC>(a, blockIdx.x * blockDim.x + threadIdx.x, chunk_sum);
+    radiance_lane_body<STORE, TRACED, OCC, K>(a, blockIdx.x * blockDim.x + threadIdx.x, chunk_sum);
 }
 
 // Many worlds in one launch.  World w owns threads [begin[w], begin[w+1]),
@@ -182,7 +272,7 @@ __global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
 // the batch.  time_seed of every world is offset by *frame (0 when null).
 // SERVED only separates the serving engine's launches from the batch API's
 // in profiles; the code is the same.
-template <int STORE, bool OCC, bool SERVED>
+template <int STORE, bool OCC, bool SERVED, int K>
 __global__ void __launch_bounds__(1024)
 radiance_batch_kernel(const RadianceArgs* args, const int* begin, int n_worlds, const int* frame) {
     extern __shared__ f3 chunk_sum[];
@@ -196,18 +286,31 @@ radiance_batch_kernel(const RadianceArgs* args, const int* begin, int n_worlds, 
     lo = __builtin_amdgcn_readfirstlane(lo);
     RadianceArgs a = args[lo];
     if (frame) a.time_seed += *frame;
-    radiance_lane_body<STORE, false, OCC>(a, (int)(blockIdx.x * blockDim.x + threadIdx.x) - begin[lo], chunk_sum);
+    radiance_lane_body<STORE, false, OCC, K>(a, (int)(blockIdx.x * blockDim.x + threadIdx.x) - begin[lo], chunk_sum);
+}
+
+template <int STORE, bool SERVED>
+static void launch_batch_k(int K, dim3 grid, dim3 block, size_t lds, hipStream_t stream, const RadianceArgs* args,
+                           const int* begin, int n, const int* frame) {
+    if (K == 4)
+        hipLaunchKernelGGL((radiance_batch_kernel<STORE, false, SERVED, 4>), grid, block, lds, stream, args, begin,
+                           n, frame);
+    else if (K == 2)
+        hipLaunchKernelGGL((radiance_batch_kernel<STORE, false, SERVED, 2>), grid, block, lds, stream, args, begin,
+                           n, frame);
+    else
+        hipLaunchKernelGGL((radiance_batch_kernel<STORE, false, SERVED, 1>), grid, block, lds, stream, args, begin,
+                           n, frame);
 }
 
 hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
-                                 int total_threads, int block, int store, bool served, const int* frame_dev,
-                                 hipStream_t stream) {
+                                 int total_threads, int block, int rays_per_lane, int store, bool served,
+                                 const int* frame_dev, hipStream_t stream) {
     if (total_threads <= 0 || n_worlds <= 0) return hipSuccess;
     const int grid = (total_threads + block - 1) / block;
-    const size_t lds = sizeof(f3) * (block / 64);
+    const size_t lds = sizeof(f3) * (block / 64) * rays_per_lane;
 #define LUMX_BATCH_LAUNCH(S, V) \
-    hipLaunchKernelGGL((radiance_batch_kernel<S, false, V>), dim3(grid), dim3(block), lds, stream, args_dev, \
-                       begin_dev, n_worlds, frame_dev)
+    launch_batch_k<S, V>(rays_per_lane, dim3(grid), dim3(block), lds, stream, args_dev, begin_dev, n_worlds, frame_dev)
     switch (store) {
         case STORE_F16: if (served) LUMX_BATCH_LAUNCH(STORE_F16, true); else LUMX_BATCH_LAUNCH(STORE_F16, false); break;
         case STORE_BF16: if (served) LUMX_BATCH_LAUNCH(STORE_BF16, true); else LUMX_BATCH_LAUNCH(STORE_BF16, false); break;
@@ -217,31 +320,48 @@ hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_
     return hipGetLastError();
 }
 
-template <int STORE, bool TRACED, bool OCC>
+// K rays per lane (K > 1 needs R a power of two, R >= K): a probe owns
+// L = R / K threads.
+template <int STORE, bool TRACED, bool OCC, int K>
 static hipError_t launch_store(const RadianceArgs& a, hipStream_t stream) {
-    const int R = a.rays;
-    int bs = R >= 256 ? R : R * (256 / R);
-    long long threads = (long long)a.n_probes * R;
+    const int R = a.rays, L = R / K;
+    int bs = L >= 256 ? L : L * (256 / L);
+    long long threads = (long long)a.n_probes * L;
     int grid = (int)((threads + bs - 1) / bs);
-    size_t lds = sizeof(f3) * (bs / 64);
+    size_t lds = sizeof(f3) * (bs / 64) * K;
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL((radiance_kernel<STORE, TRACED, OCC>), dim3(grid), dim3(bs), lds, stream, a);
+    hipLaunchKernelGGL((radiance_kernel<STORE, TRACED, OCC, K>), dim3(grid), dim3(bs), lds, stream, a);
     return hipGetLastError();
 }
 
 template <bool OCC>
-static hipError_t launch_lane(const RadianceArgs& a, int store, bool traced, hipStream_t stream) {
+static hipError_t launch_lane(const RadianceArgs& a, int store, bool traced, int K, hipStream_t stream) {
     if (traced) {
         switch (store) {
-            case STORE_F16: return launch_store<STORE_F16, true, OCC>(a, stream);
-            case STORE_BF16: return launch_store<STORE_BF16, true, OCC>(a, stream);
-            default: return launch_store<STORE_F32, true, OCC>(a, stream);
+            case STORE_F16: return launch_store<STORE_F16, true, OCC, 1>(a, stream);
+            case STORE_BF16: return launch_store<STORE_BF16, true, OCC, 1>(a, stream);
+            default: return launch_store<STORE_F32, true, OCC, 1>(a, stream);
+        }
+    }
+    // Several rays per lane: the plain (u8 palette) march only.
+    if (!OCC && K == 2) {
+        switch (store) {
+            case STORE_F16: return launch_store<STORE_F16, false, false, 2>(a, stream);
+            case STORE_BF16: return launch_store<STORE_BF16, false, false, 2>(a, stream);
+            default: return launch_store<STORE_F32, false, false, 2>(a, stream);
+        }
+    }
+    if (!OCC && K == 4) {
+        switch (store) {
+            case STORE_F16: return launch_store<STORE_F16, false, false, 4>(a, stream);
+            case STORE_BF16: return launch_store<STORE_BF16, false, false, 4>(a, stream);
+            default: return launch_store<STORE_F32, false, false, 4>(a, stream);
         }
     }
     switch (store) {
-        case STORE_F16: return launch_store<STORE_F16, false, OCC>(a, stream);
-        case STORE_BF16: return launch_store<STORE_BF16, false, OCC>(a, stream);
-        default: return launch_store<STORE_F32, false, OCC>(a, stream);
+        case STORE_F16: return launch_store<STORE_F16, false, OCC, 1>(a, stream);
+        case STORE_BF16: return launch_store<STORE_BF16, false, OCC, 1>(a, stream);
+        default: return launch_store<STORE_F32, false, OCC, 1>(a, stream);
     }
 }
 
@@ -308,8 +428,16 @@ hipError_t launch_scatter_batch(const RadianceArgs* args_dev, const int* pbegin_
 }
 
 // occ: march over the occupancy bits (global memory) instead of the u8 palette.
-hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, hipStream_t stream) {
-    return occ ? launch_lane<true>(a, store, traced, stream) : launch_lane<false>(a, store, traced, stream);
+int rays_per_lane_for(int rays, int want) {
+    if (want != 2 && want != 4) return 1;
+    if ((rays & (rays - 1)) || rays < want) return 1;
+    return want;
+}
+
+hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, int rays_per_lane,
+                           hipStream_t stream) {
+    const int K = traced || occ ? 1 : rays_per_lane_for(a.rays, rays_per_lane);
+    return occ ? launch_lane<true>(a, store, traced, K, stream) : launch_lane<false>(a, store, traced, K, stream);
 }
 
 }  // namespace lumx

@@ -270,6 +270,12 @@ int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_serve
         delete s;
         return rc;
     }
+    rc = launcher_set_shape(s->L, d->block, d->rays_per_lane);
+    if (rc) {
+        launcher_destroy(s->L);
+        delete s;
+        return rc;
+    }
     for (int k = 0; k < d->streams; ++k) {
         hipStream_t st;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {

@@ -95,10 +95,14 @@ def test_batch_frame_counter_offsets_seed(gpu_ctx):
         s.close()
 
 
-def test_batch_autotune_and_block_sizes(gpu_ctx):
+SHAPES = [(256, 1), (512, 1), (1024, 1), (256, 2), (512, 2), (1024, 4), (256, 4), (0, 0)]
+
+
+def test_batch_autotune_and_shapes(gpu_ctx):
     """The autotuner's trial launches leave every field as it was, and each
-    workgroup size it chooses from gives the single-world result (tail
-    workgroups of odd probe counts with 128 rays included)."""
+    launch shape it chooses from (workgroup size x rays per lane) gives the
+    single-world result bit for bit (tail workgroups of odd probe counts with
+    32, 64 and 128 rays included)."""
     import torch
     batch_worlds, single_worlds = [], []
     for size, rays, seed in SPECS:
@@ -111,22 +115,24 @@ def test_batch_autotune_and_block_sizes(gpu_ctx):
     try:
         params = [radiance_params(rays=r, time_seed=5 + i, alpha=0.3) for i, (_, r, _) in enumerate(SPECS)]
         before = [w.read_radiance() for w in batch_worlds]
-        block, ms = b.autotune(params, reps=2)
-        assert block in (256, 512, 1024) and ms > 0
+        block, rpl, ms = b.autotune(params, reps=2)
+        assert block in (256, 512, 1024) and rpl in (1, 2, 4) and ms > 0
         for w, f in zip(batch_worlds, before):
             assert np.array_equal(w.read_radiance(), f)
-        for blk in (256, 512, 1024, 0):
-            b.set_block(blk)
+        for blk, k in SHAPES:
+            b.set_shape(blk, k)
             b.radiance(params)
             for s, p in zip(single_worlds, params):
                 s.radiance_update(None, p)
             torch.cuda.synchronize()
             for i, (bw, sw) in enumerate(zip(batch_worlds, single_worlds)):
-                assert np.array_equal(bw.read_radiance(), sw.read_radiance()), f"block {blk} world {i}"
+                assert np.array_equal(bw.read_radiance(), sw.read_radiance()), f"shape {blk}/{k} world {i}"
             for p in params:
                 p.time_seed += 1
         with pytest.raises(Exception):
-            b.set_block(384)
+            b.set_shape(384, 1)
+        with pytest.raises(Exception):
+            b.set_shape(256, 3)
     finally:
         b.close()
         for w in batch_worlds + single_worlds:
@@ -177,7 +183,9 @@ def _chain(worlds, batch, models, placements, gbufs, outs, params):
 
 def test_graph_chain_replays_match_eager(gpu_ctx):
     import torch
-    sizes = [((16, 16, 8), 11), ((20, 16, 8), 12)]
+    # 17 x 11 x 3 = 561 blocks: 1122 bytes of block ids, not a multiple of
+    # 16, so the capture's staging of the copy list and counter is offset
+    sizes = [((16, 16, 8), 11), ((20, 16, 8), 12), ((17, 11, 3), 13)]
     rng = np.random.default_rng(77)
 
     def build():

@@ -52,6 +52,7 @@ def full_palette(world, pal):
 TUNINGS = {
     "lane": dict(variant=0),
     "lane_occ": dict(variant=2),
+    "lane_rpl4": dict(variant=0, rays_per_lane=4),
     "tiled": dict(variant=1, threads=256, probes_per_batch=16, lds_block_bits=1, lds_palette_bits=1),
     "tiled_global_bits": dict(variant=1, threads=1024, probes_per_batch=4, lds_block_bits=0, lds_palette_bits=0),
     "tiled_p32": dict(variant=1, threads=512, probes_per_batch=32, lds_block_bits=1, lds_palette_bits=0),
@@ -87,6 +88,40 @@ def test_radiance_all_probes(gpu_ctx, size, rays, seed, max_steps, tuning):
         assert np.array_equal(steps.cpu().numpy(), rsteps) or agree < 1.0
     finally:
         w.close()
+
+
+@pytest.mark.parametrize("rays", [16, 32, 64, 128, 256])
+def test_radiance_rays_per_lane_bit_exact(gpu_ctx, rays):
+    """A thread tracing 2 or 4 rays adds them in the butterfly's order: the
+    field equals the one-ray-per-thread field bit for bit (full updates and a
+    request-list subset, 1,680 probes so the last workgroup is partial), and
+    both meet the oracle."""
+    import torch
+    size = (20, 12, 7)
+    blocks, pal, mats, field = make_case(size, 31)
+    wx, wy, wz = size
+    rng = np.random.default_rng(rays)
+    flat = rng.choice(wx * wy * wz, size=333, replace=False)
+    req = np.stack([flat % wx, (flat // wx) % wy, flat // (wx * wy), np.zeros(333, int)], 1).astype(np.int32)
+    req_d = torch.from_numpy(req).cuda()
+    outs = {}
+    for k in (1, 2, 4):
+        w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+        w.set_tuning(variant=0, rays_per_lane=k)
+        try:
+            w.radiance_update(None, radiance_params(rays=rays, time_seed=9, alpha=0.2, max_steps=64))
+            w.radiance_update(req_d, radiance_params(rays=rays, time_seed=10, alpha=0.2, max_steps=64))
+            torch.cuda.synchronize()
+            outs[k] = w.read_radiance()
+            full = full_palette(w, pal)
+        finally:
+            w.close()
+    assert np.array_equal(outs[2], outs[1])
+    assert np.array_equal(outs[4], outs[1])
+    wd = orc.WorldData(blocks, full, mats)
+    ref = orc.radiance_update(wd, field, rays=rays, time_seed=9, alpha=0.2, max_steps=64)
+    ref = orc.radiance_update(wd, ref, rays=rays, time_seed=10, alpha=0.2, max_steps=64, requests=req)
+    assert rel_err(outs[1], ref).max() <= REL
 
 
 @pytest.mark.parametrize("tuning", sorted(TUNINGS))

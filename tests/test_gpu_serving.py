@@ -72,7 +72,7 @@ def run_alone(ctx, plan):
     return fields, outs
 
 
-def run_served(ctx, plan, budget=0, max_batch=8, streams=2):
+def run_served(ctx, plan, budget=0, max_batch=8, streams=2, block=0, rays_per_lane=0):
     import torch
     worlds, mats = [], []
     for i, s in enumerate(SIZES):
@@ -80,7 +80,8 @@ def run_served(ctx, plan, budget=0, max_batch=8, streams=2):
         worlds.append(w)
         mats.append(m)
     torch.cuda.synchronize()
-    srv = Server(ctx, worlds, hbm_budget=budget, max_batch=max_batch, streams=streams)
+    srv = Server(ctx, worlds, hbm_budget=budget, max_batch=max_batch, streams=streams, block=block,
+                 rays_per_lane=rays_per_lane)
     outs, tickets = [], []
     for kind, wi, a in plan:
         if kind == "r":
@@ -115,10 +116,12 @@ def _mat_rng(seed):
     return rng
 
 
-def test_served_equals_alone(gpu_ctx):
+@pytest.mark.parametrize("block,rays_per_lane", [(0, 0), (512, 2), (1024, 4)])
+def test_served_equals_alone(gpu_ctx, block, rays_per_lane):
     plan = request_plan(5, 40, _mats())
     ref_fields, ref_outs = run_alone(gpu_ctx, plan)
-    fields, outs, stats, lat = run_served(gpu_ctx, plan, max_batch=8, streams=2)
+    fields, outs, stats, lat = run_served(gpu_ctx, plan, max_batch=8, streams=2, block=block,
+                                          rays_per_lane=rays_per_lane)
     assert stats["completed"] == len(plan)
     assert stats["launches"] < len(plan)  # requests were packed together
     for a, b in zip(fields, ref_fields):
