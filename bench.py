@@ -117,8 +117,8 @@ def run_plain(args):
                 torch.cuda.synchronize()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
-                for k in range(10):
-                    w.radiance_update(None, params(k))
+                for frame in range(10):
+                    w.radiance_update(None, params(frame))
                 e1.record()
                 torch.cuda.synchronize()
                 sweep.append(dict(variant=variant, threads=threads, P=P, eb=eb, pb=pb, rays_per_lane=k,
@@ -394,12 +394,23 @@ def run_serving(args):
     # lane), chosen by the batch autotuner over max_batch random resident
     # worlds with rays from the request mix; the plain engine keeps the
     # default shape (256 threads, one ray per thread).
+    # The trial launch is one batch of the mix: a random probe subset of 1-100%
+    # per world (slices of the same permutations the requests use).
     pick = rng.choice(n_worlds, size=min(args.max_batch, n_worlds), replace=False)
-    tune_batch = Batch(ctx, [worlds[i] for i in pick])
-    tune_params = [radiance_params(rays=int(rng.choice([32, 64, 128])), time_seed=0, alpha=0.1, max_steps=64)
-                   for _ in pick]
-    served_block, served_rpl, _ = tune_batch.autotune(tune_params, reps=3)
-    tune_batch.close()
+    tune_params, tune_reqs = [], []
+    for i in pick:
+        n = sizes[i][0] * sizes[i][1] * sizes[i][2]
+        cnt = max(1, int(round(rng.uniform(0.01, 1.0) * n)))
+        off = int(rng.integers(n))
+        tune_reqs.append(perms[i][off:off + cnt])
+        tune_params.append(radiance_params(rays=int(rng.choice([32, 64, 128])), time_seed=0, alpha=0.1,
+                                           max_steps=64))
+    if args.served_shape:
+        served_block, served_rpl = (int(v) for v in args.served_shape.split(","))
+    else:
+        tune_batch = Batch(ctx, [worlds[i] for i in pick])
+        served_block, served_rpl, _ = tune_batch.autotune(tune_params, requests=tune_reqs, reps=5)
+        tune_batch.close()
     torch.cuda.synchronize()
 
     def serve(max_batch, streams, arrivals):
@@ -729,6 +740,8 @@ def main():
     ap.add_argument("--rays", type=int, default=0)
     ap.add_argument("--max-steps", type=int, default=None)
     ap.add_argument("--sweep", action="store_true", help="time every radiance launch shape (stderr)")
+    ap.add_argument("--served-shape", default="", help="serving: radiance launch shape 'block,rays_per_lane' "
+                                                        "instead of the autotuner's")
     ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
                     help="also run a plain device copy of this size (PMC byte-counter calibration)")
     args = ap.parse_args()

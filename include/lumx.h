@@ -283,12 +283,16 @@ int lumx_batch_destroy(lumx_batch_t b);
  * result equals its own lumx_radiance_update with the same arguments. */
 int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests_dev, const int* n_requests,
                         const lumx_radiance_params* params, void* stream);
-/* Autotuner of the batched radiance launch: times workgroups of 256, 512
- * and 1024 threads on every probe of every world with params[i] per world
- * (the worlds' fields are left unchanged), keeps the fastest for this
- * batch's later launches and returns it.  No reference counterpart. */
-int lumx_batch_autotune(lumx_batch_t b, const lumx_radiance_params* params, int reps, int* best_block,
-                        int* best_rays_per_lane, float* best_ms);
+/* Autotuner of the batched radiance launch: times every launch shape
+ * (workgroups of 256, 512, 1024 threads x 1, 2, 4 rays per lane) on the
+ * launch lumx_batch_radiance would make with these arguments (requests_dev
+ * NULL = every probe of every world), keeps the fastest for this batch's
+ * later launches and returns it.  The trial launches write the free field
+ * buffers only: every world's field is left unchanged.  No reference
+ * counterpart. */
+int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests_dev, const int* n_requests,
+                        const lumx_radiance_params* params, int reps, int* best_block, int* best_rays_per_lane,
+                        float* best_ms);
 /* Pins the batched radiance launch shape: workgroup size 256, 512 or 1024
  * (0 = 256) and rays per lane 1, 2 or 4 (0 = 1; a launch whose smallest
  * rays per probe is below it uses 1).  Results do not depend on either. */

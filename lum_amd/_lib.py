@@ -122,8 +122,8 @@ _SIGS = {
     "lumx_radiance_update_counted": (_I, [_P, _P, _P, _I, C.POINTER(RadianceParams), _P]),
     "lumx_world_shift_radiance": (_I, [_P, _I, _I, _I, _P]),
     "lumx_glossy": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.c_float, _I, _P, _P, _P]),
-    "lumx_batch_autotune": (_I, [_P, C.POINTER(RadianceParams), _I, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                 C.POINTER(C.c_float)]),
+    "lumx_batch_autotune": (_I, [_P, _P, _P, C.POINTER(RadianceParams), _I, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "lumx_batch_set_shape": (_I, [_P, _I, _I]),
     "lumx_map_models_device": (_I, [_P, _I, _P, _P, _P]),
     "lumx_vox_parse": (_I, [_P, C.c_size_t, _P, _P, C.c_size_t, _P, _P]),

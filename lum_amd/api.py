@@ -310,14 +310,20 @@ class Batch:
         nr = (C.c_int * n)(*[0 if r is None else int(r.shape[0]) for r in requests])
         check(self.lib.lumx_batch_radiance(self.handle, rq, nr, ps, _stream()))
 
-    def autotune(self, params, reps=3):
-        """Choose the batched radiance launch shape; returns (block, rays_per_lane, ms)."""
+    def autotune(self, params, requests=None, reps=3):
+        """Choose the batched radiance launch shape on the launch radiance()
+        would make with these arguments; returns (block, rays_per_lane, ms)."""
         n = len(self.worlds)
         ps = (RadianceParams * n)(*params)
         blk = C.c_int()
         rpl = C.c_int()
         ms = C.c_float()
-        check(self.lib.lumx_batch_autotune(self.handle, ps, reps, C.byref(blk), C.byref(rpl), C.byref(ms)))
+        rq = nr = None
+        if requests is not None:
+            rq = (C.c_void_p * n)(*[0 if r is None else r.data_ptr() for r in requests])
+            nr = (C.c_int * n)(*[0 if r is None else int(r.shape[0]) for r in requests])
+        check(self.lib.lumx_batch_autotune(self.handle, rq, nr, ps, reps, C.byref(blk), C.byref(rpl),
+                                           C.byref(ms)))
         return blk.value, rpl.value, ms.value
 
     def set_shape(self, block=0, rays_per_lane=0):

@@ -312,8 +312,9 @@ int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests, const in
                              (hipStream_t)stream, false, true);
 }
 
-int lumx_batch_autotune(lumx_batch_t b, const lumx_radiance_params* params, int reps, int* best_block,
-                        int* best_rays_per_lane, float* best_ms) {
+int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests, const int* n_requests,
+                        const lumx_radiance_params* params, int reps, int* best_block, int* best_rays_per_lane,
+                        float* best_ms) {
     if (!b || !params) return fail(LUMX_EINVAL, "null argument");
     if (reps < 1) reps = 1;
     Launcher* L = b->L;
@@ -335,13 +336,14 @@ int lumx_batch_autotune(lumx_batch_t b, const lumx_radiance_params* params, int 
             if (e != hipSuccess || rc) break;
             L->block = cand;
             L->rays_per_lane = k;
-            // Every probe of every world, written to the free field buffers
-            // only: the fields the worlds read from are left as they were.
-            rc = launcher_radiance(L, b->worlds.data(), n, nullptr, nullptr, params, true, s, false, false);
+            // The launch lumx_batch_radiance would make, written to the free
+            // field buffers only (no scatter, no flip): the fields the worlds
+            // read from are left as they were.
+            rc = launcher_radiance(L, b->worlds.data(), n, requests, n_requests, params, true, s, false, false);
             if (rc) break;
             e = hipEventRecord(e0, s);
             for (int r = 0; r < reps && rc == 0 && e == hipSuccess; ++r)
-                rc = launcher_radiance(L, b->worlds.data(), n, nullptr, nullptr, params, true, s, false, false);
+                rc = launcher_radiance(L, b->worlds.data(), n, requests, n_requests, params, true, s, false, false);
             if (e == hipSuccess) e = hipEventRecord(e1, s);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.0f;

@@ -115,6 +115,12 @@ def test_batch_autotune_and_shapes(gpu_ctx):
     try:
         params = [radiance_params(rays=r, time_seed=5 + i, alpha=0.3) for i, (_, r, _) in enumerate(SPECS)]
         before = [w.read_radiance() for w in batch_worlds]
+        wx, wy, wz = SPECS[1][0]
+        flat = np.random.default_rng(8).choice(wx * wy * wz, size=300, replace=False)
+        req = np.stack([flat % wx, (flat // wx) % wy, flat // (wx * wy), np.zeros(300, int)], 1).astype(np.int32)
+        req_d = torch.from_numpy(req).cuda()
+        block, rpl, ms = b.autotune(params, requests=[None, req_d, None, None, None], reps=2)
+        assert block in (256, 512, 1024) and rpl in (1, 2, 4) and ms > 0
         block, rpl, ms = b.autotune(params, reps=2)
         assert block in (256, 512, 1024) and rpl in (1, 2, 4) and ms > 0
         for w, f in zip(batch_worlds, before):
