@@ -287,7 +287,9 @@ int lumx_batch_radiance(lumx_batch_t b, const int32_t* const* requests_dev, cons
  * (workgroups of 256, 512, 1024 threads x 1, 2, 4 rays per lane) on the
  * launch lumx_batch_radiance would make with these arguments (requests_dev
  * NULL = every probe of every world), keeps the fastest for this batch's
- * later launches and returns it.  The trial launches write the free field
+ * later launches and returns it.  A shape other than the default (256
+ * threads, 1 ray per lane) is kept only when it is at least 3% faster, as
+ * the trial stands for launches of other sizes.  The trial launches write the free field
  * buffers only: every world's field is left unchanged.  No reference
  * counterpart. */
 int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests_dev, const int* n_requests,

@@ -53,6 +53,9 @@ struct Launcher {
     int rays_per_lane = 0;  // rays per thread of the radiance launches (0 = 1)
 };
 
+// Fraction by which an autotuned batch shape must beat the default.
+static const float kShapeMargin = 0.03f;
+
 int launcher_set_shape(Launcher* L, int block, int rays_per_lane) {
     if (block != 0 && block != 256 && block != 512 && block != 1024)
         return fail(LUMX_EINVAL, "batched workgroup size %d is not 0, 256, 512 or 1024", block);
@@ -327,7 +330,7 @@ int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests, const in
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const int saved_block = L->block, saved_k = L->rays_per_lane;
     int best_blk = saved_block, best_k = saved_k, rc = LUMX_OK;
-    float best_t = 1e30f;
+    float best_t = 1e30f, default_t = 1e30f;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     for (int k : {1, 2, 4}) {
@@ -348,12 +351,21 @@ int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests, const in
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.0f;
             if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && !rc && k == 1 && cand == 256) default_t = ms / (float)reps;
             if (e == hipSuccess && !rc && ms / (float)reps < best_t) {
                 best_t = ms / (float)reps;
                 best_blk = cand;
                 best_k = k;
             }
         }
+    }
+    // The trial launch stands for launches of other sizes and mixes: another
+    // shape replaces the default (256 threads, 1 ray per lane) only when it is
+    // clearly faster on it.
+    if (best_t > (1.0f - kShapeMargin) * default_t) {
+        best_t = default_t;
+        best_blk = 256;
+        best_k = 1;
     }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
