@@ -1298,26 +1298,32 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
     HIP_TRY(hipEventCreate(&e1));
     float best_t = 1e30f;
     lumx_radiance_tuning best_c = w->tune;
-    for (const lumx_radiance_tuning& c : cands) {
-        // Every trial writes field[cur ^ 1] only; the current field is unchanged.
-        rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c, nullptr, false);
-        if (rc) continue;  // shape does not fit this world
-        HIP_TRY(hipEventRecord(e0, s));
-        for (int r = 0; r < reps; ++r) {
+    std::vector<float> cand_ms(cands.size(), 1e30f);
+    // Two passes over the candidates, keeping each one's faster pass, so the
+    // clock ramp-up at the start does not count against the first ones.
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t ci = 0; ci < cands.size(); ++ci) {
+            const lumx_radiance_tuning& c = cands[ci];
+            // Every trial writes field[cur ^ 1] only; the current field is unchanged.
             rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c, nullptr, false);
-            if (rc) break;
+            if (rc) continue;  // shape does not fit this world
+            HIP_TRY(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) {
+                rc = radiance_launch(w, nullptr, 0, p, nullptr, nullptr, false, s, c, nullptr, false);
+                if (rc) break;
+            }
+            HIP_TRY(hipEventRecord(e1, s));
+            HIP_TRY(hipEventSynchronize(e1));
+            if (rc) continue;
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            cand_ms[ci] = std::min(cand_ms[ci], ms / (float)reps);
         }
-        HIP_TRY(hipEventRecord(e1, s));
-        HIP_TRY(hipEventSynchronize(e1));
-        if (rc) continue;
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        ms /= (float)reps;
-        if (ms < best_t) {
-            best_t = ms;
-            best_c = c;
+    for (size_t ci = 0; ci < cands.size(); ++ci)
+        if (cand_ms[ci] < best_t) {
+            best_t = cand_ms[ci];
+            best_c = cands[ci];
         }
-    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     w->tune = best_c;
