@@ -1121,13 +1121,19 @@ extern "C++" {
 int lumx::radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p) {
     if (!w || !p) return fail(LUMX_EINVAL, "null argument");
     REQUIRE_RESIDENT(w);
+    int rc = radiance_params_check(p);
+    if (rc) return rc;
+    if (req && n_req < 0) return fail(LUMX_EINVAL, "n_requests=%d invalid", n_req);
+    return LUMX_OK;
+}
+
+int lumx::radiance_params_check(const lumx_radiance_params* p) {
     if (!valid_rays(p->rays))
         return fail(LUMX_EINVAL, "rays=%d: must be a power of two <= 64 or a multiple of 64 <= 1024", p->rays);
     if (p->size < 1 || p->shift < 0 || p->shift >= p->size)
         return fail(LUMX_EINVAL, "size=%d shift=%d invalid", p->size, p->shift);
     if (!(p->max_dist > 0.0f && p->max_dist <= 65536.0f) || p->max_steps < 0)
         return fail(LUMX_EINVAL, "max_dist must be in (0, 65536] and max_steps >= 0");
-    if (req && n_req < 0) return fail(LUMX_EINVAL, "n_requests=%d invalid", n_req);
     return LUMX_OK;
 }
 }  // extern "C++"

@@ -98,6 +98,8 @@ namespace lumx {
 
 // Validation shared by single-world and batched radiance updates.
 int radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p);
+// The checks of radiance_check that do not need the world.
+int radiance_params_check(const lumx_radiance_params* p);
 // Kernel arguments of one world's radiance update (field[cur] -> field[cur ^ 1]).
 int radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p, RadianceArgs* a);
 // Kernel arguments of one glossy pass over a G-buffer.

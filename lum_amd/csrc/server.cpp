@@ -341,13 +341,20 @@ int lumx_server_submit(lumx_server_t s, const lumx_request* r, long long* ticket
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (r->world < 0 || r->world >= (int)s->worlds.size()) return fail(LUMX_EINVAL, "world id %d invalid", r->world);
+        // Everything a launch would reject is rejected here, so one bad
+        // request cannot fail the others packed into its launch.
         if (r->kind == LUMX_REQ_RADIANCE) {
             const int rays = r->params.rays;
             if (rays <= 0 || rays > 1024 || (rays & (rays - 1)))
                 return fail(LUMX_EINVAL, "served radiance rays must be a power of two <= 1024");
+            int rc = radiance_params_check(&r->params);
+            if (rc) return rc;
             if (r->requests_dev && r->n_requests < 0) return fail(LUMX_EINVAL, "n_requests invalid");
-        } else if (!r->matnorm_dev || !r->depth_dev || !r->out_dev || r->width <= 0 || r->height <= 0) {
+        } else if (!r->matnorm_dev || !r->depth_dev || !r->out_dev || r->width <= 0 || r->height <= 0 ||
+                   r->width > 16384 || r->height > 16384) {
             return fail(LUMX_EINVAL, "glossy request buffers or size invalid");
+        } else if (!(r->max_dist > 0.0f && r->max_dist <= 65536.0f) || r->max_steps < 0) {
+            return fail(LUMX_EINVAL, "glossy max_dist must be in (0, 65536] and max_steps >= 0");
         }
         q->t_submit = Clock::now();
         *ticket = (long long)s->tickets.size();

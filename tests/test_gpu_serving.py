@@ -182,3 +182,34 @@ def test_evict_restore_roundtrip(gpu_ctx):
     finally:
         w.close()
         t.close()
+
+
+def test_submit_rejects_what_a_launch_would(gpu_ctx):
+    """Requests a launch would reject are refused at submit, so they cannot
+    fail the valid requests packed with them."""
+    import torch
+    w, mats = make(gpu_ctx, (16, 16, 8), 9)
+    srv = Server(gpu_ctx, [w], max_batch=8, streams=1)
+    try:
+        wid = srv.ids[0]
+        bad = [radiance_params(rays=64, size=0), radiance_params(rays=64, size=2, shift=2),
+               radiance_params(rays=64, max_dist=0.0), radiance_params(rays=64, max_steps=-1),
+               radiance_params(rays=48)]
+        for p in bad:
+            with pytest.raises(Exception):
+                srv.submit(radiance_request(wid, p))
+        mn, dp = synth.make_gbuffer(16, 8, mats, np.random.default_rng(1), depth_hi=60.0)
+        mn_d, dp_d = torch.from_numpy(mn).cuda(), torch.from_numpy(dp).cuda()
+        out = torch.zeros((8, 16, 4), dtype=torch.float32, device="cuda")
+        with pytest.raises(Exception):
+            srv.submit(glossy_request(wid, mn_d, dp_d, camera_default(16, 8), out, 128.0, -1))
+        with pytest.raises(Exception):
+            srv.submit(glossy_request(wid, mn_d, dp_d, camera_default(16, 8), out, -1.0, 64))
+        tickets = [srv.submit(radiance_request(wid, radiance_params(rays=64, time_seed=k))) for k in range(3)]
+        tickets.append(srv.submit(glossy_request(wid, mn_d, dp_d, camera_default(16, 8), out, 128.0, 64)))
+        for t in tickets:
+            assert srv.wait(t) > 0
+        assert srv.stats()["completed"] == 4
+    finally:
+        srv.close()
+        w.close()
