@@ -476,16 +476,18 @@ void lum_submit_frame(LumRenderer instance) {
     lumx_world_info(r->world, &wd);
     const int n_blocks = wd.size[0] * wd.size[1] * wd.size[2];
     if (lumx_radiance_update_counted(r->world, r->req_dev, r->count_dev, n_blocks, &p, stream) != LUMX_OK) return;
-    lumx_world_last_pass_ms(r->world, 0, &r->radiance_ms);
-    float glossy_ms = 0.0f;
+    bool glossy_ok = false;
     if (r->gb_matnorm && r->gb_depth && r->gb_out) {
         lumx_camera cam;
         lumx_camera_default(r->gb_w, r->gb_h, &cam);
-        if (lumx_glossy(r->world, r->gb_matnorm, r->gb_depth, r->gb_w, r->gb_h, &cam, 128.0f, 0, r->gb_out,
-                        nullptr, stream) == LUMX_OK)
-            lumx_world_last_pass_ms(r->world, 2, &glossy_ms);
+        glossy_ok = lumx_glossy(r->world, r->gb_matnorm, r->gb_depth, r->gb_w, r->gb_h, &cam, 128.0f, 0, r->gb_out,
+                                nullptr, stream) == LUMX_OK;
     }
-    hipStreamSynchronize((hipStream_t)stream);
+    // Pass times once the frame's work is queued (each read waits for its pass).
+    lumx_world_last_pass_ms(r->world, 0, &r->radiance_ms);
+    float glossy_ms = 0.0f;
+    if (glossy_ok) lumx_world_last_pass_ms(r->world, 2, &glossy_ms);
+    (void)hipStreamSynchronize((hipStream_t)stream);
     r->ts[0] = 0.0;
     r->ts[1] = map_ms;
     r->ts[2] = r->ts[1] + r->radiance_ms;

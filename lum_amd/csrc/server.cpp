@@ -215,19 +215,27 @@ static void worker_loop(lumx_server* s) {
                 it = s->pending.erase(it);
             }
             if (!f.reqs.empty()) {
+                int rc = LUMX_OK;
                 if (s->free_events.empty()) {
-                    hipEvent_t e;
-                    hipEventCreateWithFlags(&e, hipEventDisableTiming);
-                    s->free_events.push_back(e);
+                    hipEvent_t e = nullptr;
+                    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess)
+                        s->free_events.push_back(e);
+                    else
+                        rc = fail(LUMX_EDEVICE, "completion event creation failed");
                 }
-                f.done = s->free_events.back();
-                s->free_events.pop_back();
-                f.stream = s->next_stream;
-                s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
-                int rc = launch_flight(s, f);
-                s->stats.launches++;
+                if (rc == LUMX_OK) {
+                    f.done = s->free_events.back();
+                    s->free_events.pop_back();
+                    f.stream = s->next_stream;
+                    s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
+                    rc = launch_flight(s, f);
+                    s->stats.launches++;
+                }
                 if (rc != LUMX_OK) {
-                    // Record the failure on the requests; their worlds are released.
+                    // Record the failure on the requests; their worlds are
+                    // released once whatever part of the launch was queued
+                    // has finished.
+                    if (f.done) (void)hipStreamSynchronize(s->streams[f.stream]);
                     const Clock::time_point now = Clock::now();
                     for (size_t j = 0; j < f.reqs.size(); ++j) {
                         f.reqs[j]->status = rc;
@@ -235,7 +243,7 @@ static void worker_loop(lumx_server* s) {
                         f.reqs[j]->done = true;
                         s->worlds[f.worlds[j]].busy--;
                     }
-                    s->free_events.push_back(f.done);
+                    if (f.done) s->free_events.push_back(f.done);
                     s->cv_done.notify_all();
                 } else {
                     s->flights.push_back(std::move(f));
