@@ -282,19 +282,28 @@ LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_
     int it = 0;
     int cur = 0;
     while (true) {
+        // Iterations begun, counted before any lane can leave, so the count
+        // stays wave-uniform (scalar) on the GPU.
+        ++it;
         fraction += 0.5f;
         f3 pos = o + d * fraction;
         const int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
         // Block coordinate ivec3(p) / 16 (truncating) as trunc(p / 16): the
         // same integer, p / 16 being exact, without the signed division.
-        const int bx = (int)(pos.x * 0.0625f), by = (int)(pos.y * 0.0625f), bz = (int)(pos.z * 0.0625f);
+        const f3 ps = pos * 0.0625f;
+        const int bx = (int)ps.x, by = (int)ps.y, bz = (int)ps.z;
         const bool inb =
             (unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz;
         const int blk = inb ? block_id(w, bx, by, bz) : 0;
         cur = voxel_in_block(w, blk, ix, iy, iz);
         r.coarse++;
         if (blk == 0) {
-            f3 box = {(float)(bx * 16) + corner.x, (float)(by * 16) + corner.y, (float)(bz * 16) + corner.z};
+            // (float)(b * 16) + corner as trunc(p / 16) * 16 + corner: the same
+            // integers, exact in float, in one FMA per axis.  (Lanes whose b
+            // would overflow are outside the world and leave the loop below
+            // without a hit, so their fraction is never read.)
+            f3 box = {fmaf(truncf(ps.x), 16.0f, corner.x), fmaf(truncf(ps.y), 16.0f, corner.y),
+                      fmaf(truncf(ps.z), 16.0f, corner.z)};
             f3 tmp = (-pos) * inv;
             f3 bc = box * inv + tmp;
             float f = fminf(bc.x, fminf(bc.y, bc.z));
@@ -309,7 +318,7 @@ LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_
         // coordinate past the far edge, since trunc(p / 16) < W <=> p < 16 W.
         if (!inb || fminf(pos.x, fminf(pos.y, pos.z)) < 0.0f) break;
         if (fraction >= max_dist) break;
-        if (max_steps > 0 && ++it >= max_steps) break;
+        if (max_steps > 0 && it >= max_steps) break;
     }
     if (!r.hit) return r;
 
@@ -403,6 +412,9 @@ LUMX_HD RayHit cast_ray_occ(const OccView& w, f3 o, f3 d, float max_dist, int ma
     float fraction = 0.0f;
     int it = 0;
     while (true) {
+        // Iterations begun, counted before any lane can leave, so the count
+        // stays wave-uniform (scalar) on the GPU.
+        ++it;
         fraction += 0.5f;
         f3 pos = o + d * fraction;
         int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
@@ -425,7 +437,7 @@ LUMX_HD RayHit cast_ray_occ(const OccView& w, f3 o, f3 d, float max_dist, int ma
         if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || pos.x >= ext_x || pos.y >= ext_y || pos.z >= ext_z)
             break;
         if (fraction >= max_dist) break;
-        if (max_steps > 0 && ++it >= max_steps) break;
+        if (max_steps > 0 && it >= max_steps) break;
     }
     if (!r.hit) return r;
 
