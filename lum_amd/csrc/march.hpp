@@ -132,6 +132,8 @@ template <> struct Texel<STORE_BF16> {
 };
 
 // a * b for operands below 2^24 (a full-rate 24-bit multiply on the GPU).
+LUMX_HD unsigned umin_(unsigned a, unsigned b) { return a < b ? a : b; }
+
 LUMX_HD unsigned mul24(unsigned a, unsigned b) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __umul24(a, b);
@@ -206,14 +208,23 @@ LUMX_HD MatF get_mat(const WorldView& w, int v) {
     return r;
 }
 
-// i mod n in [0, n); one conditional add or subtract covers i in [-n, 2n).
+// i mod n in [0, n) for n in [1, 2^30).  For i in [-n, 2n), u = i + n lies
+// in [0, 3n) and two conditional subtracts finish it; min(u, u - n) over
+// unsigned values is u - n exactly when u >= n.  Other i divide.
 LUMX_HD int wrap_i(int i, int n) {
-    int r = i < 0 ? i + n : (i >= n ? i - n : i);
-    if ((unsigned)r >= (unsigned)n) {
-        r = i % n;
-        r = r < 0 ? r + n : r;
+    unsigned u = (unsigned)i + (unsigned)n;
+    if (u >= 3u * (unsigned)n) {
+        const int r = i % n;
+        return r < 0 ? r + n : r;
     }
-    return r;
+    u = umin_(u, u - (unsigned)n);
+    return (int)umin_(u, u - (unsigned)n);
+}
+
+// (x + 1) mod n for x in [0, n).
+LUMX_HD int wrap_next(int x, int n) {
+    const unsigned u = (unsigned)x + 1u;
+    return (int)umin_(u, u - (unsigned)n);
 }
 
 // Trilinear LINEAR filter with REPEAT wrap at texel-space coordinate bp
@@ -223,9 +234,10 @@ LUMX_HD f3 sample_texel(const void* field, int wx, int wy, int wz, f3 bp) {
     float fx0 = floorf(bp.x), fy0 = floorf(bp.y), fz0 = floorf(bp.z);
     float tx = bp.x - fx0, ty = bp.y - fy0, tz = bp.z - fz0;
     int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    int x0 = wrap_i(ix, wx), x1 = wrap_i(ix + 1, wx);
-    int y0 = wrap_i(iy, wy), y1 = wrap_i(iy + 1, wy);
-    int z0 = wrap_i(iz, wz), z1 = wrap_i(iz + 1, wz);
+    // REPEAT: the second tap of each axis is the first one's successor mod n.
+    int x0 = wrap_i(ix, wx), x1 = wrap_next(x0, wx);
+    int y0 = wrap_i(iy, wy), y1 = wrap_next(y0, wy);
+    int z0 = wrap_i(iz, wz), z1 = wrap_next(z0, wz);
     // Operands below 2^24 (axes <= 4096, wy * wz <= 2^24): 24-bit multiplies.
     const unsigned r00 = mul24((unsigned)wx, (unsigned)y0 + mul24((unsigned)wy, (unsigned)z0));
     const unsigned r10 = mul24((unsigned)wx, (unsigned)y1 + mul24((unsigned)wy, (unsigned)z0));

@@ -110,7 +110,10 @@ def test_host_built_sample_radiance_matches_oracle(mh):
     size = (7, 5, 4)
     field = rng.random((size[2], size[1], size[0], 4)).astype(np.float32)
     n = 4000
+    # texel coordinates from below -n to beyond 2n on every axis: both the
+    # two-subtract REPEAT wrap and its division fallback
     x = rng.uniform(-40, 150, (n, 3)).astype(np.float32)
+    x[n // 2:] = rng.uniform(-260, 400, (n - n // 2, 3)).astype(np.float32)
     nr = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
     out = np.zeros((n, 3), np.float32)
     mh.mh_sample_radiance(_p(field), size[0], size[1], size[2], n, _p(x), _p(nr), _p(out))
