@@ -186,11 +186,16 @@ LUMX_HD int voxel_in_block(const WorldView& w, int b, int ix, int iy, int iz) {
 }
 
 // GetBlock + palette fetch for an integer voxel position.
+// The block coordinate i / 16 (truncating) lies in [0, W) exactly when i lies
+// in [-15, 16 W), and there it equals max(i, 0) >> 4: one add and compare for
+// the bounds, a max and a shift for the fetch, instead of a signed division.
+LUMX_HD bool block_axis_in(int i, int w) { return (unsigned)i + 15u < (unsigned)(16 * w + 15); }
+LUMX_HD int block_axis(int i) { return (i > 0 ? i : 0) >> 4; }
+
 LUMX_HD int get_voxel(const WorldView& w, int ix, int iy, int iz, int* blk_out) {
-    int bx = ix / 16, by = iy / 16, bz = iz / 16;
     int b = 0;
-    if ((unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz)
-        b = block_id(w, bx, by, bz);
+    if (block_axis_in(ix, w.wx) && block_axis_in(iy, w.wy) && block_axis_in(iz, w.wz))
+        b = block_id(w, block_axis(ix), block_axis(iy), block_axis(iz));
     *blk_out = b;
     return voxel_in_block(w, b, ix, iy, iz);
 }
