@@ -417,7 +417,7 @@ def run_serving(args):
         shape = (served_block, served_rpl) if max_batch > 1 else (0, 0)
         srv = Server(ctx, worlds, hbm_budget=args.hbm_budget_gb * (1 << 30), max_batch=max_batch, streams=streams,
                      block=shape[0], rays_per_lane=shape[1])
-        lat, wall = srv.run_trace(build(srv), arrivals)
+        lat, wall = srv.run_trace(build(srv)[:len(arrivals)], arrivals)
         st = srv.stats()
         srv.close()
         return lat, wall, st

@@ -431,6 +431,8 @@ class Server:
 
     def run_trace(self, reqs, arrivals):
         n = len(reqs)
+        if len(arrivals) != n:
+            raise ValueError("one arrival time per request")
         arr = (Request * max(n, 1))(*reqs)
         at = (C.c_double * max(n, 1))(*arrivals)
         lat = (C.c_double * max(n, 1))()
