@@ -134,14 +134,38 @@ __device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int
     return light;
 }
 
-// v[i] += v[i ^ s] for s = width/2 .. 1 (width a power of two <= 64).
-__device__ __forceinline__ f3 butterfly(f3 v, int width) {
-    for (int s = width >> 1; s >= 1; s >>= 1) {
-        float ox = __shfl_xor(v.x, s);
-        float oy = __shfl_xor(v.y, s);
-        float oz = __shfl_xor(v.z, s);
-        v = v + f3{ox, oy, oz};
+// One butterfly stage, v[i] + v[i ^ S], with every lane of the wave active.
+// S = 32: a swap of the two 32-lane halves hands each lane both its own and
+// its partner's value (in some order), and their sum is that stage's sum;
+// S = 16..4: swizzle in XOR mode; S = 2, 1: DPP quad permutes, which the
+// compiler folds into the add.
+template <int S>
+__device__ __forceinline__ float bfly_stage(float v) {
+    if constexpr (S == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+        return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    } else if constexpr (S >= 4) {
+        return v + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F | (S << 10)));
+    } else {
+        constexpr int quad_perm = S == 2 ? 0x4E : 0xB1;  // [2,3,0,1] : [1,0,3,2]
+        return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), quad_perm, 0xF, 0xF, true));
     }
+}
+
+template <int S>
+__device__ __forceinline__ f3 bfly_stage3(f3 v) {
+    return {bfly_stage<S>(v.x), bfly_stage<S>(v.y), bfly_stage<S>(v.z)};
+}
+
+// v[i] += v[i ^ s] for s = width/2 .. 1 (width a power of two <= 64; the
+// whole wave executes it).
+__device__ __forceinline__ f3 butterfly(f3 v, int width) {
+    if (width > 32) v = bfly_stage3<32>(v);
+    if (width > 16) v = bfly_stage3<16>(v);
+    if (width > 8) v = bfly_stage3<8>(v);
+    if (width > 4) v = bfly_stage3<4>(v);
+    if (width > 2) v = bfly_stage3<2>(v);
+    if (width > 1) v = bfly_stage3<1>(v);
     return v;
 }
 

@@ -95,7 +95,8 @@ def test_radiance_rays_per_lane_bit_exact(gpu_ctx, rays):
     """A thread tracing 2 or 4 rays adds them in the butterfly's order: the
     field equals the one-ray-per-thread field bit for bit (full updates and a
     request-list subset, 1,680 probes so the last workgroup is partial), and
-    both meet the oracle."""
+    that equals the oracle's bit for bit (butterfly stages of every width,
+    chunk sums beyond 64 rays)."""
     import torch
     size = (20, 12, 7)
     blocks, pal, mats, field = make_case(size, 31)
@@ -121,7 +122,8 @@ def test_radiance_rays_per_lane_bit_exact(gpu_ctx, rays):
     wd = orc.WorldData(blocks, full, mats)
     ref = orc.radiance_update(wd, field, rays=rays, time_seed=9, alpha=0.2, max_steps=64)
     ref = orc.radiance_update(wd, ref, rays=rays, time_seed=10, alpha=0.2, max_steps=64, requests=req)
-    assert rel_err(outs[1], ref).max() <= REL
+    # FP32 follows the oracle's arithmetic and reduction order exactly
+    assert np.array_equal(outs[1], ref)
 
 
 @pytest.mark.parametrize("tuning", sorted(TUNINGS))
