@@ -551,7 +551,7 @@ def cpu_baseline_serving(trace, hosts, perm_host, sizes, seconds=10.0):
 
 
 def run_chain(args):
-    """the 48x48x16 world plus 32 models of 16^3-64^3
+    """The chain workload: the 48x48x16 world plus 32 models of 16^3-64^3
     (30% fill, uniform rotations) mapped, every probe updated (64 rays), and a
     1920x1080 glossy march (max 128 steps) from a synthetic G-buffer, FP32.
     A step is the whole map -> radiance -> glossy chain on one stream."""
@@ -645,7 +645,7 @@ def run_chain(args):
 
 
 def run_precision(args):
-    """a 256x256x32-block world (2,097,152 probes),
+    """The precision workload: a 256x256x32-block world (2,097,152 probes),
     128 rays per probe, max 96 steps, 8 accumulated frames, radiance stored in
     FP32, FP16 and BF16 (FP32 march and accumulation).  Each is compared with
     the FP64 CPU oracle run on the same inputs, on a seeded 4,096-probe

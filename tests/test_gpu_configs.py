@@ -1,6 +1,6 @@
 """Parity at the benchmark configurations' full sizes.
 
-the 48x48x16 world with 32 models of 16^3-64^3 (30% fill,
+The chain workload: the 48x48x16 world with 32 models of 16^3-64^3 (30% fill,
 uniform rotations) mapped bit-exactly, every probe updated (64 rays, max 64
 steps) and a 1920x1080 glossy march (max 128 steps), each against the CPU
 oracle on the same inputs.
