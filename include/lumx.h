@@ -263,8 +263,10 @@ int lumx_diffuse(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_
  * vox_read_scene) in load_mesh / load_block / extract_palette and builds its
  * Material table in extract_palette_mem (src/renderer/src/load_stuff.cpp:
  * 233-342).  Parses a .vox file image: size_out[3] gets the first distinct
- * non-empty model's size; voxels (x-fastest colour indices, 0 = empty) is
- * written when voxels_capacity >= size_x * size_y * size_z; materials (256 x
+ * non-empty model's size; voxels (x-fastest colour indices, 0 = empty), when
+ * non-NULL, needs voxels_capacity >= size_x * size_y * size_z (else
+ * LUMX_EINVAL, with size_out set: call once with NULL to size it); models
+ * are at most 1024 voxels a side and 2 GiB dense in total; materials (256 x
  * {r, g, b, a, emission, roughness}) is written when non-NULL;
  * n_models_out (optional) gets the number of distinct non-empty models. */
 int lumx_vox_parse(const void* data, size_t bytes, int* size_out, uint8_t* voxels, size_t voxels_capacity,

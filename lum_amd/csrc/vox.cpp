@@ -153,6 +153,7 @@ extern "C" int lumx_vox_parse(const void* data, size_t bytes, int* size_out, uin
     bool have_imap = false;
     std::vector<Model> models;   // empty voxel vector = a model with no voxels
     uint32_t sx = 0, sy = 0, sz = 0;
+    size_t model_bytes = 0;      // all dense models held so far
     std::vector<std::pair<std::string, std::string>> dict;
 
     while (r.left() >= 12) {
@@ -168,11 +169,16 @@ extern "C" int lumx_vox_parse(const void* data, size_t bytes, int* size_out, uin
             r.u32(&sz);
         } else if (id == chunk_id('X', 'Y', 'Z', 'I')) {
             if (!(sx && sy && sz)) return lumx::fail(LUMX_EINVAL, ".vox: XYZI chunk before a SIZE chunk");
-            if ((uint64_t)sx * sy * sz > (1ull << 30)) return lumx::fail(LUMX_EINVAL, ".vox: model too large");
+            // Models are at most 1024 voxels a side (the largest a lumx model
+            // takes), and all of a file's models together at most 2 GiB dense.
+            if (sx > 1024 || sy > 1024 || sz > 1024)
+                return lumx::fail(LUMX_EINVAL, ".vox: model %ux%ux%u larger than 1024 a side", sx, sy, sz);
             uint32_t n = 0;
             r.u32(&n);
             Model m{sx, sy, sz, {}};
             if (n != 0) {
+                model_bytes += (size_t)sx * sy * sz;
+                if (model_bytes > (1ull << 31)) return lumx::fail(LUMX_EINVAL, ".vox: models too large in total");
                 m.v.assign((size_t)sx * sy * sz, 0);
                 const size_t avail = r.left() / 4 < n ? r.left() / 4 : n;
                 const uint8_t* q = r.p + r.off;
@@ -286,6 +292,11 @@ extern "C" int lumx_vox_parse(const void* data, size_t bytes, int* size_out, uin
     size_out[0] = (int)m0.sx;
     size_out[1] = (int)m0.sy;
     size_out[2] = (int)m0.sz;
-    if (voxels && voxels_capacity >= m0.v.size()) std::memcpy(voxels, m0.v.data(), m0.v.size());
+    if (voxels) {
+        if (voxels_capacity < m0.v.size())
+            return lumx::fail(LUMX_EINVAL, ".vox: voxel buffer holds %zu bytes, the model needs %zu", voxels_capacity,
+                              m0.v.size());
+        std::memcpy(voxels, m0.v.data(), m0.v.size());
+    }
     return LUMX_OK;
 }

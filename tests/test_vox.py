@@ -191,3 +191,17 @@ def test_not_a_vox_file_is_an_error():
     rc = lumx_parse(b"RIFF" + b"\x00" * 64)[0]
     assert rc != 0
     assert lumx_parse(vox_file(b""))[0] != 0               # no model
+
+
+def test_oversized_models_and_short_buffers_are_errors():
+    lib = load()
+    # a side over 1024 voxels
+    big = chunk(b"SIZE", struct.pack("<III", 2000, 1, 1)) + chunk(b"XYZI", struct.pack("<I", 1) + bytes([0, 0, 0, 1]))
+    assert lumx_parse(vox_file(big))[0] != 0
+    # a voxel buffer smaller than the model: an error, with the size reported
+    data, m = SYNTH["default_palette"]
+    buf = C.create_string_buffer(data, len(data))
+    size = (C.c_int * 3)()
+    small = np.zeros(m.size - 1, np.uint8)
+    rc = lib.lumx_vox_parse(buf, len(data), size, small.ctypes.data_as(C.c_void_p), small.size, None, None)
+    assert rc != 0 and tuple(size) == (m.shape[2], m.shape[1], m.shape[0])
