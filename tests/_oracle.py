@@ -22,6 +22,8 @@ class OrcRadParams(C.Structure):
 
 
 _lib = None
+_lib_fma = None
+ORACLE_FMA_SO = os.path.join(ORACLE_DIR, "liblum_oracle_fma.so")
 
 
 def build():
@@ -35,6 +37,17 @@ def load():
             build()
         _lib = C.CDLL(ORACLE_SO)
     return _lib
+
+
+def load_contracted():
+    """The oracle built with FMA contraction on (a fusing shader compiler's
+    stand-in); only for measuring how much contraction moves results."""
+    global _lib_fma
+    if _lib_fma is None:
+        if not os.path.exists(ORACLE_FMA_SO):
+            build()
+        _lib_fma = C.CDLL(ORACLE_FMA_SO)
+    return _lib_fma
 
 
 def _p(a):
@@ -98,7 +111,7 @@ def sample_radiance(field, x, n):
 
 
 def radiance_update(world, field, requests=None, rays=64, time_seed=0, size=1, shift=0, alpha=0.1,
-                    max_dist=128.0, max_steps=0, threads=8, traced=False, fp64=False):
+                    max_dist=128.0, max_steps=0, threads=8, traced=False, fp64=False, contracted=False):
     """Returns the new field (and per-ray keys / steps when traced).
     requests None = every probe in x-fastest order."""
     wx, wy, wz = world.size
@@ -120,7 +133,8 @@ def radiance_update(world, field, requests=None, rays=64, time_seed=0, size=1, s
     else:
         fin = np.ascontiguousarray(field, np.float32)
         fout = np.zeros_like(fin)
-        load().orc_radiance_update_f32(C.byref(world.c), _p(fin), _p(fout), _p(req), C.byref(p), threads, kp, sp)
+        lib = load_contracted() if contracted else load()
+        lib.orc_radiance_update_f32(C.byref(world.c), _p(fin), _p(fout), _p(req), C.byref(p), threads, kp, sp)
     if traced:
         return fout, keys[:n_rays], steps[: 2 * n_rays].reshape(-1, 2)
     return fout
