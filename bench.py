@@ -318,6 +318,9 @@ def run_batched(args):
     return out
 
 
+REQUESTS_PER_STEP = 20
+
+
 def run_serving(args):
     """200 resident worlds (log-uniform 16x16x8..128x128x32 blocks, FP16
     radiance).  A Poisson stream of 80% radiance updates (random 1-100% probe
@@ -363,7 +366,12 @@ def run_serving(args):
     torch.cuda.synchronize()
 
     # request trace
-    n_req = args.requests
+    # A serving step is REQUESTS_PER_STEP requests of the trace: --steps K
+    # times exactly 20 K requests (4000 by default), --warmup W serves 20 W
+    # untimed requests first.  --requests overrides the timed count.
+    n_req = args.requests if args.requests else REQUESTS_PER_STEP * args.steps
+    n_warm = REQUESTS_PER_STEP * args.warmup
+    steps_out = n_req / REQUESTS_PER_STEP if args.requests else args.steps
     trace = []  # (kind, world, offset, count, rays)
     for i in range(n_req):
         wi = int(rng.integers(n_worlds))
@@ -423,7 +431,8 @@ def run_serving(args):
         return lat, wall, st
 
     burst = [0.0] * n_req
-    serve(1, 1, burst[: min(200, n_req)])  # warm-up
+    if n_warm > 0:
+        serve(1, 1, burst[: min(n_warm, n_req)])  # warm-up
     plain_lat_s, plain_wall_s, plain_st = serve(1, 1, burst)
     served_lat_s, served_wall_s, served_st = serve(args.max_batch, args.streams, burst)
     rate = 0.7 * n_req / plain_wall_s
@@ -470,9 +479,9 @@ def run_serving(args):
         "value": value,
         "unit": "probe-rays/s",
         "n_gpus": 1,
-        "steps": n_req,
-        "warmup": 200,
-        "ms_per_step": served_wall_s * 1e3 / n_req,
+        "steps": steps_out,
+        "warmup": min(n_warm, n_req) // REQUESTS_PER_STEP,
+        "ms_per_step": served_wall_s * 1e3 / steps_out,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -481,6 +490,7 @@ def run_serving(args):
         "config": {"workload": "serving: %d resident worlds log-uniform 16x16x8..128x128x32 blocks, FP16 radiance; "
                                "%d requests, 80%% radiance (1-100%% probe subset, 32/64/128 rays, max 64 steps), "
                                "20%% glossy 640x360 (max 128 steps)" % (n_worlds, n_req),
+                   "requests": n_req, "requests_per_step": REQUESTS_PER_STEP,
                    "max_batch": args.max_batch, "streams": args.streams,
                    "served_shape": {"block": served_block, "rays_per_lane": served_rpl},
                    "hbm_budget_gb": args.hbm_budget_gb, "probe_rays": probe_rays, "glossy_pixels": pixels},
@@ -729,7 +739,8 @@ def main():
     ap.add_argument("--workload", default="serving", choices=["serving", "plain", "batched", "chain", "precision"])
     ap.add_argument("--frames", type=int, default=8)
     ap.add_argument("--serving-worlds", type=int, default=200)
-    ap.add_argument("--requests", type=int, default=4000)
+    ap.add_argument("--requests", type=int, default=0,
+                    help="serving: timed requests (default 20 per step, 4000 at the default --steps)")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--streams", type=int, default=2)
     ap.add_argument("--hbm-budget-gb", type=int, default=64)
