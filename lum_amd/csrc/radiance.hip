@@ -3,13 +3,14 @@
 // Restates shaders/radiance.comp:361-413 (main), :51-66 (hash33/Random3D),
 // :316-347 (trace_ray) and :349-354 (store_probe) of the reference.
 //
-// Launch shape: one thread per ray.  A probe's R rays are R consecutive
-// lanes; with R <= 64 a wave64 holds 64/R whole probes, with R a multiple
-// of 64 a probe spans R/64 whole waves of one workgroup.  The probe mean is
-// reduced in a fixed order (butterfly over min(R,64) lanes, then chunk sums
-// in chunk order, then one division by R) so results do not depend on
-// timing.  Jacobi semantics: rays sample field_in, the probe is written to
-// field_out.
+// Launch shape: one thread per ray (or K = 2, 4 rays per thread, traced one
+// after another).  A probe's R / K threads are consecutive lanes; with
+// R / K <= 64 a wave64 holds whole probes, otherwise a probe spans whole
+// waves of one workgroup.  The probe mean is reduced in a fixed order
+// (butterfly over min(R,64) rays, then chunk sums in chunk order, then one
+// division by R) for every K, so results depend neither on timing nor on
+// the launch shape.  Jacobi semantics: rays sample field_in, the probe is
+// written to field_out.
 #include <type_traits>
 
 #include "march.hpp"
