@@ -39,7 +39,7 @@ __device__ __forceinline__ void glossy_pixel(const GlossyArgs& a, const int px, 
     const float dn = dot3(normal, dir);
     dir = dir - (2.0f * dn) * normal;
 
-    RayHit h = cast_ray<true>(w, origin, dir, a.max_dist, a.max_steps);
+    RayHit h = cast_ray(w, origin, dir, a.max_dist, a.max_steps);
     if (h.hit) {
         MatF hm = get_mat(w, h.voxel);
         f3 o2 = origin + h.fraction * dir;

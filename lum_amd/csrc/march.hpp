@@ -172,13 +172,10 @@ LUMX_HD uint32_t fast_div(uint32_t n, Divisor v) {
 // (axes <= 4096, wy * wz <= 2^24) and the byte offset is formed in 32 bits
 // (worlds hold at most 2^28 blocks), so the load takes a scalar base plus a
 // 32-bit vector offset.
-LUMX_HD unsigned block_lin(const WorldView& w, int bx, int by, int bz) {
-    return (unsigned)bx + mul24((unsigned)w.wx, (unsigned)by + mul24((unsigned)w.wy, (unsigned)bz));
-}
-LUMX_HD int block_at(const WorldView& w, unsigned lin) {
+LUMX_HD int block_id(const WorldView& w, int bx, int by, int bz) {
+    const unsigned lin = (unsigned)bx + mul24((unsigned)w.wx, (unsigned)by + mul24((unsigned)w.wy, (unsigned)bz));
     return gld((const int16_t*)((const char*)w.blocks + (lin << 1)), 0);
 }
-LUMX_HD int block_id(const WorldView& w, int bx, int by, int bz) { return block_at(w, block_lin(w, bx, by, bz)); }
 
 // voxel_in_palette: voxel (ix, iy, iz) mod 16 of block b.
 LUMX_HD int voxel_in_block(const WorldView& w, int b, int ix, int iy, int iz) {
@@ -287,12 +284,6 @@ struct RayHit {
 
 // CastRay_fast: coarse 0.5-voxel march with empty-block skipping, then up to
 // five voxel-DDA refinement steps from origin + dir*(fraction - 1.5).
-// CACHE_BLOCK keeps the last block id a lane fetched and reuses it while the
-// ray stays in that block (the world is read-only during a pass): one fetch
-// per block instead of one per 0.5-voxel sample inside non-empty blocks.  It
-// costs a compare per sample, so it pays where fetches, not instructions,
-// bound the pass (the glossy march over scattered pixels).
-template <bool CACHE_BLOCK = false>
 LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_steps) {
     RayHit r;
     r.hit = 0;
@@ -307,8 +298,6 @@ LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_
     float fraction = 0.0f;
     int it = 0;
     int cur = 0;
-    unsigned last_lin = ~0u;  // CACHE_BLOCK: the block last fetched, and its id
-    int last_blk = 0;
     while (true) {
         // Iterations begun, counted before any lane can leave, so the count
         // stays wave-uniform (scalar) on the GPU.
@@ -322,19 +311,7 @@ LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_
         const int bx = (int)ps.x, by = (int)ps.y, bz = (int)ps.z;
         const bool inb =
             (unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz;
-        int blk = 0;
-        if (inb) {
-            const unsigned lin = block_lin(w, bx, by, bz);
-            if (CACHE_BLOCK && lin == last_lin) {
-                blk = last_blk;
-            } else {
-                blk = block_at(w, lin);
-                if (CACHE_BLOCK) {
-                    last_lin = lin;
-                    last_blk = blk;
-                }
-            }
-        }
+        const int blk = inb ? block_id(w, bx, by, bz) : 0;
         cur = voxel_in_block(w, blk, ix, iy, iz);
         r.coarse++;
         if (blk == 0) {

@@ -10,8 +10,7 @@ using namespace lumx;
 extern "C" {
 
 // One cast per ray: info = {hit, voxel, coarse, refine, key}, fn = {fraction, normal.xyz}.
-// occ: 0 = cast_ray over the u8 palette, 1 = cast_ray_occ over the occupancy
-// bits, 2 = cast_ray with the block-id cache of the glossy march.
+// occ != 0 marches the occupancy bits (cast_ray_occ) instead of the u8 palette.
 void mh_cast_rays(const int16_t* blocks, const uint8_t* palette, const float* mats, int wx, int wy, int wz,
                   int palette_slots, int block0_empty, const uint32_t* empty_bits, const uint32_t* pal_bits,
                   int occ, int n, const float* o, const float* d, float max_dist, int max_steps, int* info,
@@ -38,9 +37,7 @@ void mh_cast_rays(const int16_t* blocks, const uint8_t* palette, const float* ma
     for (int i = 0; i < n; ++i) {
         const f3 oo = {o[3 * i], o[3 * i + 1], o[3 * i + 2]};
         const f3 dd = {d[3 * i], d[3 * i + 1], d[3 * i + 2]};
-        RayHit h = occ == 1   ? cast_ray_occ(ov, oo, dd, max_dist, max_steps)
-                   : occ == 2 ? cast_ray<true>(w, oo, dd, max_dist, max_steps)
-                              : cast_ray(w, oo, dd, max_dist, max_steps);
+        RayHit h = occ ? cast_ray_occ(ov, oo, dd, max_dist, max_steps) : cast_ray(w, oo, dd, max_dist, max_steps);
         info[5 * i + 0] = h.hit;
         info[5 * i + 1] = h.voxel;
         info[5 * i + 2] = h.coarse;

@@ -56,7 +56,7 @@ def rays(rng, size, n):
     return o.astype(np.float32), d.astype(np.float32)
 
 
-@pytest.mark.parametrize("occ", [0, 1, 2])
+@pytest.mark.parametrize("occ", [0, 1])
 @pytest.mark.parametrize("seed,size,max_steps", [(1, (12, 10, 6), 0), (2, (20, 16, 8), 64), (3, (9, 9, 9), 7)])
 def test_host_built_cast_ray_matches_oracle(mh, occ, seed, size, max_steps):
     rng = np.random.default_rng(seed)
