@@ -731,7 +731,7 @@ def run_precision(args):
     }
 
 
-def main():
+def make_parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -755,7 +755,11 @@ def main():
                                                         "instead of the autotuner's")
     ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
                     help="also run a plain device copy of this size (PMC byte-counter calibration)")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = make_parser().parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the hot path has no multi-GPU form here); got --gpus {args.gpus}",
               file=sys.stderr)
