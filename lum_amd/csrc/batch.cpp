@@ -337,11 +337,12 @@ int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests, const in
         if (k > 1 && rays_per_lane_for(min_r, k) != k) continue;
         for (int blk : {256, 512, 1024}) cands.push_back({blk, k, 1e30f});
     }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EventPair ev;
     const int saved_block = L->block, saved_k = L->rays_per_lane;
     int rc = LUMX_OK;
-    hipError_t e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
+    hipError_t e = hipEventCreate(&ev.a);
+    if (e == hipSuccess) e = hipEventCreate(&ev.b);
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     // Two passes over the candidates, keeping each one's faster pass, so the
     // clock ramp-up at the start does not count against the first ones.
     for (int pass = 0; pass < 2; ++pass)
@@ -363,8 +364,6 @@ int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests, const in
             if (e == hipSuccess && !rc) e = hipEventElapsedTime(&ms, e0, e1);
             if (e == hipSuccess && !rc) c.ms = std::min(c.ms, ms / (float)reps);
         }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
     L->block = saved_block;
     L->rays_per_lane = saved_k;
     if (rc) return rc;

@@ -1299,9 +1299,10 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
                 c.lds_palette_bits = pb;
                 cands.push_back(c);
             }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    EventPair ev;
+    HIP_TRY(hipEventCreate(&ev.a));
+    HIP_TRY(hipEventCreate(&ev.b));
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     float best_t = 1e30f;
     lumx_radiance_tuning best_c = w->tune;
     std::vector<float> cand_ms(cands.size(), 1e30f);
@@ -1330,8 +1331,6 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
             best_t = cand_ms[ci];
             best_c = cands[ci];
         }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     w->tune = best_c;
     if (best) *best = best_c;
     if (best_ms) *best_ms = best_t;

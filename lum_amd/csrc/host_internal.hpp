@@ -26,6 +26,18 @@ void clear_error();
                                 __FILE__, __LINE__);                                               \
     } while (0)
 
+// Two timing events destroyed on every return path.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 }  // namespace lumx
 
 struct lumx_context {
