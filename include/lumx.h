@@ -209,7 +209,11 @@ int lumx_world_palette_counter(lumx_world_t w, int* out);
 
 /* radiance.comp: requests_dev = int32x4 (x, y, z, _) probe list in HBM, or
  * NULL for every probe in x-fastest order (n_requests ignored).  Reads the
- * current field, writes a new one (Jacobi), then makes it current. */
+ * current field, writes a new one (Jacobi), then makes it current.  A list
+ * dispatches n_requests / size probes, at most the world's block count (the
+ * capacity of the reference's request buffer; more is LUMX_EINVAL).  Entries
+ * outside the world are skipped; where several dispatched entries name one
+ * probe, the last of them is the one stored. */
 int lumx_radiance_update(lumx_world_t w, const int32_t* requests_dev, int n_requests,
                          const lumx_radiance_params* p, void* stream);
 /* Same, also writing per-ray hit keys (0 = miss) and {coarse, refine} sample
@@ -226,15 +230,17 @@ int lumx_world_get_tuning(lumx_world_t w, lumx_radiance_tuning* out);
  * counterpart. */
 int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int reps,
                            lumx_radiance_tuning* best, float* best_ms);
-/* Probe selection of update_radiance on the current (blockified) world.
- * Specials outside the world are an error; a special repeated in the list
- * is taken once. */
+/* Probe selection of update_radiance on the current (blockified) world: the
+ * probes whose 27 clamped neighbours hold a non-zero block, x-fastest, then
+ * every special whose probe the grid did not select, repeats included, in
+ * order.  The list ends at the world's block count (the capacity of the
+ * reference's request buffer), so requests_host needs room for n_blocks
+ * int32x4 entries.  Specials outside the world are an error. */
 int lumx_select_probes(lumx_world_t w, const int32_t* special_host, int n_special,
                        int32_t* requests_host, int* n_out);
 /* The same selection on the device, stream-ordered after the last map:
- * requests_dev (room for n_blocks int32x4 entries: a probe appears at most
- * once) receives the list, *count_dev its length.  Specials outside the world
- * are skipped. */
+ * requests_dev (room for n_blocks int32x4 entries) receives the list,
+ * *count_dev its length.  Specials outside the world are skipped. */
 int lumx_select_probes_device(lumx_world_t w, const int32_t* special_dev, int n_special,
                               int32_t* requests_dev, int32_t* count_dev, void* stream);
 /* lumx_radiance_update over a list whose length is read on the device
@@ -271,6 +277,16 @@ int lumx_diffuse(lumx_world_t w, const uint8_t* matnorm_dev, const float* depth_
  * n_models_out (optional) gets the number of distinct non-empty models. */
 int lumx_vox_parse(const void* data, size_t bytes, int* size_out, uint8_t* voxels, size_t voxels_capacity,
                    float* materials, int* n_models_out);
+/* ---- world file ("scene") ----
+ * load_scene (src/renderer/src/load_stuff.cpp:187-217): a file image of a
+ * 12-byte int32 header (stored size Sx, Sy, Sz) and int16 blocks indexed
+ * x + Sx * y + Sx * Sy * z.  Into blocks (x-fastest, world_size[0..2]) it
+ * writes the stored blocks of the region clamp(S, 0, world_size), each
+ * clamped to [0, static_blocks]; blocks outside that region keep their
+ * values.  Fewer than 12 bytes clear the whole world.  A file holding fewer
+ * blocks than its header's region needs is LUMX_EINVAL (the reference
+ * asserts).  Host only; lum_load_world_file reads files with it. */
+int lumx_scene_read(const void* data, size_t bytes, const int world_size[3], int static_blocks, int16_t* blocks);
 /* Lum's default camera (internal_types.hpp:54-66) for a frame size. */
 int lumx_camera_default(int frame_w, int frame_h, lumx_camera* out);
 

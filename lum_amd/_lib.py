@@ -129,6 +129,7 @@ _SIGS = {
     "lumx_vox_parse": (_I, [_P, C.c_size_t, _P, _P, C.c_size_t, _P, _P]),
     "lumx_diffuse": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.POINTER(Lightmap), _P, _P]),
     "lumx_camera_default": (_I, [_I, _I, C.POINTER(Camera)]),
+    "lumx_scene_read": (_I, [_P, C.c_size_t, C.POINTER(C.c_int), _I, _P]),
     "lumx_world_last_pass_ms": (_I, [_P, _I, C.POINTER(C.c_float)]),
     "lumx_world_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "lumx_world_get_tuning": (_I, [_P, C.POINTER(Tuning)]),

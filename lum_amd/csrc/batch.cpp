@@ -198,6 +198,8 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
         int nr = req ? n_requests[i] : 0;
         rc = radiance_args(w, req, nr, &params[i], &args[i]);
         if (rc) return rc;
+        // The scatter that applies a subset update consumes (and resets) the claims.
+        if (args[i].staged && commit) args[i].claim = w->upd_claim;
         begin[i] = (int)total;
         total += round_up((size_t)args[i].n_probes * (args[i].rays / K), (size_t)block);
         if (total > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batch too large");

@@ -187,43 +187,60 @@ bool parse_vox_file(const char* path, int size[3], std::vector<uint8_t>* voxels,
     return lumx_vox_parse(buf.data(), buf.size(), size, voxels->data(), voxels->size(), mats, nullptr) == LUMX_OK;
 }
 
+// load_scene (load_stuff.cpp:187-217) into the instance's origin world; an
+// unreadable file reads as empty (readFileBuffer returns no bytes).
 void load_scene_file(Renderer* r, const char* path) {
     if (!r->world || !path) return;
-    FILE* f = std::fopen(path, "rb");
-    const int wx = r->settings.world_size[0], wy = r->settings.world_size[1], wz = r->settings.world_size[2];
-    std::vector<int16_t> world((size_t)wx * wy * wz, 0);
-    if (f) {
-        std::vector<char> buf;
-        char tmp[65536];
-        size_t n;
-        while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
-        std::fclose(f);
-        // 12-byte ivec3 header, then int16 x-fastest blocks, clamped to
-        // [0, static] (load_stuff.cpp:187-217).
-        if (buf.size() >= 12) {
-            int hs[3];
-            std::memcpy(hs, buf.data(), 12);
-            const int16_t* stored = (const int16_t*)(buf.data() + 12);
-            size_t n_stored = (buf.size() - 12) / 2;
-            int rx = std::min(std::max(hs[0], 0), wx), ry = std::min(std::max(hs[1], 0), wy),
-                rz = std::min(std::max(hs[2], 0), wz);
-            for (int z = 0; z < rz; ++z)
-                for (int y = 0; y < ry; ++y)
-                    for (int x = 0; x < rx; ++x) {
-                        size_t idx = (size_t)x + (size_t)hs[0] * y + (size_t)hs[0] * hs[1] * z;
-                        if (idx >= n_stored) continue;
-                        int16_t b = stored[idx];
-                        b = std::min<int16_t>(std::max<int16_t>(b, 0), (int16_t)r->settings.static_block_palette_size);
-                        world[x + (size_t)wx * (y + (size_t)wy * z)] = b;
-                    }
-        }
-    }
+    std::vector<uint8_t> buf;
+    if (!read_file(path, &buf)) buf.clear();
+    const int* ws = r->settings.world_size;
+    std::vector<int16_t> world((size_t)ws[0] * ws[1] * ws[2], 0);
+    if (lumx_world_get_blocks(r->world, world.data(), 0) != LUMX_OK) return;
+    if (lumx_scene_read(buf.data(), buf.size(), ws, r->settings.static_block_palette_size, world.data()) != LUMX_OK)
+        return;
     lumx_world_set_blocks(r->world, world.data());
 }
 
 }  // namespace
 
 extern "C" {
+
+int lumx_scene_read(const void* data, size_t bytes, const int world_size[3], int static_blocks,
+                    int16_t* blocks) {
+    if ((!data && bytes) || !world_size || !blocks) return lumx::fail(LUMX_EINVAL, "null argument");
+    const int wx = world_size[0], wy = world_size[1], wz = world_size[2];
+    if (wx <= 0 || wy <= 0 || wz <= 0) return lumx::fail(LUMX_EINVAL, "world size invalid");
+    const size_t n_world = (size_t)wx * wy * wz;
+    // Fewer bytes than the header: the world is cleared (origin_world.set(0)).
+    if (bytes < 12) {
+        std::fill(blocks, blocks + n_world, (int16_t)0);
+        return LUMX_OK;
+    }
+    int hs[3];
+    std::memcpy(hs, data, 12);
+    const uint8_t* stored = (const uint8_t*)data + 12;
+    const size_t n_stored = (bytes - 12) / 2;
+    // size2read = clamp(stored size, 0, world size); the stored blocks are
+    // indexed with the stored size, x + Sx * y + Sx * Sy * z.
+    const int rx = std::min(std::max(hs[0], 0), wx), ry = std::min(std::max(hs[1], 0), wy),
+              rz = std::min(std::max(hs[2], 0), wz);
+    if (rx > 0 && ry > 0 && rz > 0) {
+        const long long last = (long long)(rx - 1) + (long long)hs[0] * (ry - 1) + (long long)hs[0] * hs[1] * (rz - 1);
+        if (last >= (long long)n_stored)
+            return lumx::fail(LUMX_EINVAL, "scene holds %zu blocks, its %dx%dx%d header needs more", n_stored, hs[0],
+                              hs[1], hs[2]);
+    }
+    const int16_t hi = (int16_t)std::min(std::max(static_blocks, 0), 32767);
+    for (int z = 0; z < rz; ++z)
+        for (int y = 0; y < ry; ++y)
+            for (int x = 0; x < rx; ++x) {
+                const size_t idx = (size_t)x + (size_t)hs[0] * y + (size_t)hs[0] * hs[1] * z;
+                int16_t b;
+                std::memcpy(&b, stored + 2 * idx, 2);
+                blocks[x + (size_t)wx * (y + (size_t)wy * z)] = std::min(std::max(b, (int16_t)0), hi);
+            }
+    return LUMX_OK;
+}
 
 LumRenderer lum_create_instance(const LumSettings* settings) {
     Renderer* r = new Renderer();

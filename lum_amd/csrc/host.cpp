@@ -290,6 +290,7 @@ static hipError_t world_alloc_device(lumx_world* w) {
     chk(hipMalloc(&w->sel_bits, lumx::select_bits_bytes(w->n_blocks)));
     chk(hipMalloc(&w->sel_tiles, lumx::select_tiles(w->n_blocks) * sizeof(int)));
     chk(hipMalloc(&w->sel_claim, nb * sizeof(int)));
+    chk(hipMalloc(&w->upd_claim, nb * sizeof(int)));
     chk(hipMalloc(&w->origin_dev, nb * sizeof(int16_t)));
     chk(hipMalloc(&w->blockify_state, 4 * sizeof(int)));
     if (e != hipSuccess) return e;
@@ -297,6 +298,7 @@ static hipError_t world_alloc_device(lumx_world* w) {
     w->origin_dirty = true;
     static_assert(sizeof(int) == 4, "claim words");
     chk(hipMemset(w->sel_claim, lumx::select_claim_idle() & 0xff, nb * sizeof(int)));
+    chk(hipMemset(w->upd_claim, 0xff, nb * sizeof(int)));  // -1: no request
     chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
     chk(hipMemset(w->pal_bits, 0, slots * 128 * sizeof(uint32_t)));
     chk(hipMemset(w->batch_counter, 0, 2 * sizeof(unsigned int)));
@@ -313,7 +315,7 @@ static size_t world_device_bytes(const lumx_world* w) {
     const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
     return nb * 2 + slots * 4096 + 256 * 6 * 4 + slots * 4 + 2 * nb * w->texel_bytes + ((nb + 31) / 32) * 4 +
            slots * 128 * 4 + 8 + lumx::select_bits_bytes(w->n_blocks) +
-           lumx::select_tiles(w->n_blocks) * sizeof(int) + nb * sizeof(int) + nb * 2 + 16;
+           lumx::select_tiles(w->n_blocks) * sizeof(int) + 2 * nb * sizeof(int) + nb * 2 + 16;
 }
 
 static void world_free_device(lumx_world* w) {
@@ -329,6 +331,8 @@ static void world_free_device(lumx_world* w) {
     hipFree(w->sel_bits);
     hipFree(w->sel_tiles);
     hipFree(w->sel_claim);
+    hipFree(w->upd_claim);
+    w->upd_claim = nullptr;
     hipFree(w->origin_dev);
     hipFree(w->blockify_state);
     hipFree(w->blockify_scratch);
@@ -870,12 +874,16 @@ static int blockify(lumx_world* w, const lumx_placement& p) {
 // its own.  Models of one layer write disjoint palette bytes, so a layer is
 // one launch (in groups of kMapGroup) and the result equals submission-order
 // mapping.  An origin world holding dynamic ids could alias two blocks to one
-// slot; it maps one model per launch.
+// slot; it maps one model per launch.  A model whose dispatch box reaches
+// below 0 on some axis writes through aliased coordinates into block 0
+// (map.hip), outside its box: it gets a layer of its own, mapped as one launch
+// per sign class in increasing class order.
 static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
                          hipStream_t s) {
     const int st = w->desc.static_blocks;
     std::vector<MapArgs> args(std::max(n, 1));
     std::vector<int> layer(std::max(n, 1), 0);
+    std::vector<char> negative(std::max(n, 1), 0);
     if (w->origin_ids_dirty) {
         w->origin_has_dynamic = false;
         for (int16_t b : w->origin)
@@ -900,10 +908,18 @@ static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const
         a.shift_x = p.shift[0]; a.shift_y = p.shift[1]; a.shift_z = p.shift[2];
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 4; ++c) a.m[r * 4 + c] = p.minv[c * 4 + r];
+        a.cls = 7;
+        a.tiles_x = (a.nx + 15) / 16;
+        a.tiles_y = a.ny / 4;
+        if ((long long)a.tiles_x * a.tiles_y * (a.nz / 4) >= (1ll << 28))
+            return fail(LUMX_EINVAL, "model %d dispatch too large", i);
+        a.div_tiles_x = make_divisor((uint32_t)std::max(a.tiles_x, 1));
+        a.div_tiles_y = make_divisor((uint32_t)std::max(a.tiles_y, 1));
+        negative[i] = a.shift_x < 0 || a.shift_y < 0 || a.shift_z < 0;
         layer[i] = 0;
         for (int j = 0; j < i; ++j) {
             const MapArgs& b = args[j];
-            const bool overlap = aliasing ||
+            const bool overlap = aliasing || negative[i] || negative[j] ||
                 (a.shift_x < b.shift_x + b.nx && b.shift_x < a.shift_x + a.nx && a.shift_y < b.shift_y + b.ny &&
                  b.shift_y < a.shift_y + a.ny && a.shift_z < b.shift_z + b.nz && b.shift_z < a.shift_z + a.nz);
             if (overlap) layer[i] = std::max(layer[i], layer[j] + 1);
@@ -917,12 +933,30 @@ static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const
         for (int i = 0; i <= n; ++i) {
             const bool take = i < n && layer[i] == l;
             if (take) {
-                const long long threads = (long long)args[i].nx * args[i].ny * args[i].nz;
-                if (threads <= 0) continue;
-                const long long groups = (threads + 255) / 256;
+                const MapArgs& a = args[i];
+                const long long groups = (long long)a.tiles_x * a.tiles_y * (a.nz / 4);
+                if (groups <= 0) continue;
                 if ((long long)g.begin[g.n] + groups > (1ll << 31) - 1)
                     return fail(LUMX_EINVAL, "model %d dispatch too large", i);
-                g.m[g.n] = args[i];
+                if (negative[i]) {
+                    // Alone in its layer: one launch per sign class present.
+                    const int lo[3] = {a.shift_x, a.shift_y, a.shift_z};
+                    const int hi[3] = {a.shift_x + a.nx, a.shift_y + a.ny, a.shift_z + a.nz};
+                    for (int cls = 0; cls < 8; ++cls) {
+                        bool present = true;
+                        for (int k = 0; k < 3; ++k) present &= (cls >> k & 1) ? hi[k] > 0 : lo[k] < 0;
+                        if (!present) continue;
+                        MapGroup one;
+                        one.n = 1;
+                        one.m[0] = a;
+                        one.m[0].cls = cls;
+                        one.begin[0] = 0;
+                        one.begin[1] = (int)groups;
+                        HIP_TRY(launch_map_group(one, s));
+                    }
+                    continue;
+                }
+                g.m[g.n] = a;
                 g.begin[g.n + 1] = g.begin[g.n] + (int)groups;
                 g.n++;
             }
@@ -1124,6 +1158,12 @@ int lumx::radiance_check(const lumx_world* w, const int32_t* req, int n_req, con
     int rc = radiance_params_check(p);
     if (rc) return rc;
     if (req && n_req < 0) return fail(LUMX_EINVAL, "n_requests=%d invalid", n_req);
+    // A subset update stages probe i's texel in slot i of the free field, which
+    // holds one texel per probe: the reference's request buffer likewise holds
+    // one entry per probe (internal_render.cpp:198-200).
+    if (req && n_req / p->size > w->n_blocks)
+        return fail(LUMX_EINVAL, "%d requests at stride %d dispatch more probes than the world's %d", n_req,
+                    p->size, w->n_blocks);
     return LUMX_OK;
 }
 
@@ -1205,6 +1245,8 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
     a.n_requests_dev = count_dev;
     a.ray_keys = keys;
     a.ray_steps = steps;
+    // The scatter that applies a subset update consumes (and resets) the claims.
+    if (a.staged && apply) a.claim = w->upd_claim;
     if (tune.variant == 0 || tune.variant == 2) {
         HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, tune.variant == 2, tune.rays_per_lane, s));
         if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
@@ -1357,7 +1399,7 @@ int lumx_select_probes_device(lumx_world_t w, const int32_t* special_dev, int n_
                               int32_t* count_dev, void* stream) {
     if (!w || !out_dev || !count_dev || n_special < 0 || (n_special > 0 && !special_dev))
         return fail(LUMX_EINVAL, "bad argument");
-    if (n_special >= lumx::select_claim_idle()) return fail(LUMX_EINVAL, "too many special requests");
+    if (n_special > (1 << 30)) return fail(LUMX_EINVAL, "too many special requests");
     REQUIRE_RESIDENT(w);
     HIP_TRY(hipSetDevice(w->ctx->device));
     hipStream_t s = pick_stream(w->ctx, stream);
@@ -1372,7 +1414,6 @@ int lumx_select_probes_device(lumx_world_t w, const int32_t* special_dev, int n_
     a.tile_counts = w->sel_tiles;
     a.special = (const int4*)special_dev;
     a.n_special = n_special;
-    a.claim = w->sel_claim;
     a.out = (int4*)out_dev;
     a.count = count_dev;
     HIP_TRY(lumx::launch_select(a, s));
@@ -1435,14 +1476,16 @@ int lumx_select_probes(lumx_world_t w, const int32_t* special, int n_special, in
                     ++n;
                 }
             }
+    // Every special whose probe the grid did not select is appended, repeats
+    // included (the reference marks only grid probes in `set`); the list ends
+    // at one entry per probe, the capacity of the reference's request buffer.
     for (int i = 0; i < n_special; ++i) {
         int x = special[4 * i], y = special[4 * i + 1], z = special[4 * i + 2];
         if (!in_world(w, x, y, z)) return fail(LUMX_EINVAL, "special request %d outside the world", i);
         size_t idx = x + (size_t)wx * (y + (size_t)wy * z);
-        if (!set[idx]) {
+        if (!set[idx] && n < w->n_blocks) {
             int32_t* o = out + 4 * n;
             o[0] = x; o[1] = y; o[2] = z; o[3] = special[4 * i + 3];
-            set[idx] = 1;
             ++n;
         }
     }

@@ -81,6 +81,8 @@ struct lumx_world {
     int* sel_tiles = nullptr;               // device probe selection: per-tile counts / offsets
     int* sel_claim = nullptr;               // per block claim word shared by probe selection and
                                             // device blockify (idle between calls)
+    int* upd_claim = nullptr;               // per probe: last request of a subset update that
+                                            // named it (-1 between updates)
     // Device blockify.
     int16_t* origin_dev = nullptr;          // device copy of the origin world
     bool origin_dirty = true;               // origin changed since its last upload

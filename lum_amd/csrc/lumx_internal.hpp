@@ -22,6 +22,9 @@ struct RadianceArgs {
     Divisor div_x, div_xy;  // wx and wx * wy, for probe coordinates of x-fastest indices
     int staged;             // 1: probe i's new texel goes to field_out[i] (a subset update,
                             // applied in place by the scatter kernel), 0: to its own texel
+    int* claim;             // staged updates followed by a scatter: per texel, the highest
+                            // request index that updated it (-1 between updates), so the
+                            // last of duplicate requests is the one applied; nullptr = none
     int time_seed;
     int size;
     int shift;
@@ -60,6 +63,9 @@ struct MapArgs {
     int nx, ny, nz;         // threads per axis (4 * groups)
     int shift_x, shift_y, shift_z;
     float m[12];            // rows 0..2 of the inverse transform: m[r*4 + c]
+    int cls;                // sign class of the threads this launch maps (7: all coordinates >= 0)
+    int tiles_x, tiles_y;   // 16 x 4 x 4 thread tiles per axis: (nx + 15) / 16, ny / 4
+    Divisor div_tiles_x, div_tiles_y;
 };
 
 // Models of one map launch (kernel argument, at most kMapGroup models).
@@ -114,9 +120,8 @@ struct SelectArgs {
     int* tile_counts;           // per tile: count, then offset
     const int4* special;        // special requests (x, y, z, w)
     int n_special;
-    int* claim;                 // per probe: first special index, idle value otherwise
-    int4* out;                  // n + n_special entries
-    int* count;                 // entries written
+    int4* out;                  // n entries
+    int* count;                 // entries written (at most n)
 };
 
 // Field shift (probes.hip): texel p + shift lands on p inside the destination box.
@@ -172,7 +177,6 @@ size_t radiance_tiled_lds_bytes(const RadianceArgs& a, const RadianceTune& t);
 hipError_t launch_block_bits(const int16_t* blocks, int n_blocks, uint32_t* bits, hipStream_t stream);
 hipError_t launch_palette_bits(const uint8_t* palette, int first_slot, int n_slots, uint32_t* bits,
                                hipStream_t stream);
-hipError_t launch_map(const MapArgs& a, hipStream_t stream);
 hipError_t launch_map_group(const MapGroup& g, hipStream_t stream);
 hipError_t launch_exec_copies(const CopyArgs& a, hipStream_t stream);
 hipError_t launch_glossy(const GlossyArgs& a, int store, bool traced, hipStream_t stream);

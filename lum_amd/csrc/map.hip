@@ -1,11 +1,23 @@
 // map.hip - model->world voxelization and palette slot copies (gfx950).
 //
-// map_kernel restates shaders/map.comp:33-68 of the reference: one thread per
+// map_group_kernel restates shaders/map.comp:33-68 of the reference: one thread per
 // candidate world voxel of the model's dispatch box, FP32 inverse transform
 // (unfused, rows evaluated ((m0*x + m1*y) + m2*z) + m3), truncation to model
 // voxel, predicated byte store into the palette slot the blockified world
-// assigns to that block.  Threads whose world coordinate is negative write
-// nothing (there the reference's writes alias and race).
+// assigns to that block.
+//
+// Threads whose world coordinate g = gid + shift is negative on an axis
+// alias, as in the shader: the coordinate truncates toward zero (g = -1 gives
+// world voxel 0), its block is the truncating iw / 16 (block 0 for
+// -16 < iw < 0) and its voxel within the block is iw % 16 with the sign of
+// the divisor, iw & 15 (OpSMod).  So they write voxels of block 0 that
+// non-negative threads write too.  Where several threads of one model write
+// one voxel, the one with the highest dispatch index (z, then y, then x) is
+// the value kept.  Per axis the map from g to the voxel is one-to-one within
+// g < 0 and within g >= 0, and a thread with g >= 0 has the higher index, so
+// a model's threads split into sign classes (bit 2: z >= 0, bit 1: y >= 0,
+// bit 0: x >= 0) that write disjoint voxels, and mapping the classes in
+// increasing order keeps the highest index.  MapArgs::cls selects one class.
 //
 // copy_kernel restates the transfer part of exec_copies
 // (src/renderer/src/internal_render.cpp:1375-1476): every slot allocated by
@@ -14,12 +26,10 @@
 
 namespace lumx {
 
-__device__ __forceinline__ void map_thread(const MapArgs& a, long long t) {
-    if (t >= (long long)a.nx * a.ny * a.nz) return;
-    const int gx = (int)(t % a.nx);
-    const int gy = (int)((t / a.nx) % a.ny);
-    const int gz = (int)(t / ((long long)a.nx * a.ny));
-    if (gx + a.shift_x < 0 || gy + a.shift_y < 0 || gz + a.shift_z < 0) return;
+// Thread (gx, gy, gz) of the model's dispatch box.
+__device__ __forceinline__ void map_thread(const MapArgs& a, int gx, int gy, int gz) {
+    const int cls = (gx + a.shift_x >= 0 ? 1 : 0) | (gy + a.shift_y >= 0 ? 2 : 0) | (gz + a.shift_z >= 0 ? 4 : 0);
+    if (cls != a.cls) return;
 
     const float wvx = ((float)gx + 0.5f) + (float)a.shift_x;
     const float wvy = ((float)gy + 0.5f) + (float)a.shift_y;
@@ -49,18 +59,28 @@ __device__ __forceinline__ void map_thread(const MapArgs& a, long long t) {
     a.palette[off] = v;
 }
 
-__global__ void __launch_bounds__(256) map_kernel(MapArgs a) {
-    map_thread(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 // Several models whose write boxes are disjoint in one launch: model i owns
 // workgroups [begin[i], begin[i + 1]).  Disjoint boxes write disjoint palette
-// bytes, so the result equals mapping them one after another.
+// bytes, so the result equals mapping them one after another.  A workgroup of
+// 256 threads covers a 16 x 4 x 4 tile of its model's dispatch box (x
+// fastest, so a row of 16 threads stores into one 16-byte palette row); the
+// tile is found by 32-bit divisions of the workgroup index, once per
+// workgroup on the scalar unit.
 __global__ void __launch_bounds__(256) map_group_kernel(MapGroup g) {
     const int b = blockIdx.x;
     int i = 0;
     while (i + 1 < g.n && g.begin[i + 1] <= b) ++i;
-    map_thread(g.m[i], (long long)(b - g.begin[i]) * 256 + threadIdx.x);
+    const MapArgs& a = g.m[i];
+    const uint32_t t = (uint32_t)(b - g.begin[i]);
+    const uint32_t r = fast_div(t, a.div_tiles_x);
+    const uint32_t bx = t - r * (uint32_t)a.tiles_x;
+    const uint32_t bz = fast_div(r, a.div_tiles_y);
+    const uint32_t by = r - bz * (uint32_t)a.tiles_y;
+    const int gx = (int)(bx * 16u + (threadIdx.x & 15u));
+    const int gy = (int)(by * 4u + ((threadIdx.x >> 4) & 3u));
+    const int gz = (int)(bz * 4u + (threadIdx.x >> 6));
+    if (gx >= a.nx) return;
+    map_thread(a, gx, gy, gz);
 }
 
 // One workgroup of 256 threads per slot, 16 B per thread.
@@ -71,14 +91,6 @@ __global__ void __launch_bounds__(256) copy_kernel(CopyArgs a) {
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
     if (src != 0) v = ((const uint4*)(a.palette + ((size_t)src << 12)))[threadIdx.x];
     dst[threadIdx.x] = v;
-}
-
-hipError_t launch_map(const MapArgs& a, hipStream_t stream) {
-    long long total = (long long)a.nx * a.ny * a.nz;
-    if (total <= 0) return hipSuccess;
-    int grid = (int)((total + 255) / 256);
-    hipLaunchKernelGGL(map_kernel, dim3(grid), dim3(256), 0, stream, a);
-    return hipGetLastError();
 }
 
 hipError_t launch_map_group(const MapGroup& g, hipStream_t stream) {

@@ -9,13 +9,14 @@
 //
 //   select_flags_kernel   one workgroup per tile of 2048 probes: the flags as
 //                         64-bit wave ballots (one word per 64 probes) and
-//                         the tile's count; extra workgroups claim every
-//                         special request whose probe is not flagged with
-//                         atomicMin of its index (first occurrence wins);
+//                         the tile's count;
 //   select_scan_kernel    one workgroup: exclusive scan of the tile counts,
-//                         then the specials in order (kept = claimed by this
-//                         index), written after the grid probes, claims
-//                         reset, total count stored;
+//                         then the specials in order (kept = inside the world
+//                         and its probe not flagged; repeats are kept, as the
+//                         reference marks only grid probes as taken), written
+//                         after the grid probes up to one entry per probe
+//                         (the capacity of the reference's request buffer),
+//                         count stored;
 //   select_scatter_kernel one workgroup per tile: each flagged probe goes to
 //                         tile offset + rank within the tile.
 //
@@ -56,36 +57,28 @@ __device__ __forceinline__ int neighbour_sum(const int16_t* blocks, int wx, int 
 __global__ __launch_bounds__(kSelThreads) void select_flags_kernel(SelectArgs a) {
     const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (tile < a.n_tiles) {
-        __shared__ int s_count[kSelThreads / 64];
-        int count = 0;
-        for (int k = 0; k < kSelTile / kSelThreads; ++k) {
-            const int p = tile * kSelTile + k * kSelThreads + threadIdx.x;
-            bool flag = false;
-            if (p < a.n) {
-                const int x = p % a.wx, y = (p / a.wx) % a.wy, z = p / (a.wx * a.wy);
-                flag = neighbour_sum(a.blocks, a.wx, a.wy, a.wz, x, y, z) > 0;
-            }
-            const unsigned long long m = __ballot(flag);
-            // Word c of the tile holds probes tile*2048 + 64c .. +63, c = 4k + wave.
-            if (lane == 0) a.bits[(size_t)tile * (kSelTile / 64) + k * (kSelThreads / 64) + wave] = m;
-            count += __popcll(m);
+    __shared__ int s_count[kSelThreads / 64];
+    int count = 0;
+    for (int k = 0; k < kSelTile / kSelThreads; ++k) {
+        const int p = tile * kSelTile + k * kSelThreads + threadIdx.x;
+        bool flag = false;
+        if (p < a.n) {
+            const int x = p % a.wx, y = (p / a.wx) % a.wy, z = p / (a.wx * a.wy);
+            flag = neighbour_sum(a.blocks, a.wx, a.wy, a.wz, x, y, z) > 0;
         }
-        if (lane == 0) s_count[wave] = count;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int t = 0;
-            for (int i = 0; i < kSelThreads / 64; ++i) t += s_count[i];
-            a.tile_counts[tile] = t;
-        }
-        return;
+        const unsigned long long m = __ballot(flag);
+        // Word c of the tile holds probes tile*2048 + 64c .. +63, c = 4k + wave,
+        // so probe p's flag is bit p % 64 of word p / 64.
+        if (lane == 0) a.bits[(size_t)tile * (kSelTile / 64) + k * (kSelThreads / 64) + wave] = m;
+        count += __popcll(m);
     }
-    const int i = (tile - a.n_tiles) * kSelThreads + threadIdx.x;
-    if (i >= a.n_special) return;
-    const int4 q = a.special[i];
-    if ((unsigned)q.x >= (unsigned)a.wx || (unsigned)q.y >= (unsigned)a.wy || (unsigned)q.z >= (unsigned)a.wz) return;
-    if (neighbour_sum(a.blocks, a.wx, a.wy, a.wz, q.x, q.y, q.z) > 0) return;
-    atomicMin(&a.claim[q.x + a.wx * (q.y + a.wy * q.z)], i);
+    if (lane == 0) s_count[wave] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int i = 0; i < kSelThreads / 64; ++i) t += s_count[i];
+        a.tile_counts[tile] = t;
+    }
 }
 
 // Inclusive scan of v over the workgroup (blockDim.x a multiple of 64, at
@@ -129,26 +122,20 @@ __global__ __launch_bounds__(1024) void select_scan_kernel(SelectArgs a) {
         const int i = base + threadIdx.x;
         int4 q = make_int4(0, 0, 0, 0);
         int keep = 0;
-        int idx = 0;
         if (i < a.n_special) {
             q = a.special[i];
             if ((unsigned)q.x < (unsigned)a.wx && (unsigned)q.y < (unsigned)a.wy && (unsigned)q.z < (unsigned)a.wz) {
-                idx = q.x + a.wx * (q.y + a.wy * q.z);
-                // Only unflagged probes were claimed, by their first special.  A
-                // duplicate reads either the winner's index or the idle value
-                // written back below, never its own index.
-                keep = a.claim[idx] == i;
+                const int p = q.x + a.wx * (q.y + a.wy * q.z);
+                keep = !((a.bits[p >> 6] >> (p & 63)) & 1ull);  // not selected by the grid
             }
         }
         int total;
         const int incl = group_scan(keep, s_wave, &total);
-        if (keep) {
-            a.out[running + incl - 1] = q;
-            a.claim[idx] = kClaimIdle;
-        }
+        const int pos = running + incl - 1;
+        if (keep && pos < a.n) a.out[pos] = q;
         running += total;
     }
-    if (threadIdx.x == 0) a.count[0] = running;
+    if (threadIdx.x == 0) a.count[0] = min(running, a.n);
 }
 
 __global__ __launch_bounds__(kSelThreads) void select_scatter_kernel(SelectArgs a) {
@@ -204,9 +191,8 @@ size_t select_bits_bytes(int n_probes) { return select_tiles(n_probes) * (kSelTi
 int select_claim_idle() { return kClaimIdle; }
 
 hipError_t launch_select(const SelectArgs& a, hipStream_t stream) {
-    const int special_groups = (a.n_special + kSelThreads - 1) / kSelThreads;
-    if (a.n_tiles + special_groups > 0)
-        hipLaunchKernelGGL(select_flags_kernel, dim3(a.n_tiles + special_groups), dim3(kSelThreads), 0, stream, a);
+    if (a.n_tiles > 0)
+        hipLaunchKernelGGL(select_flags_kernel, dim3(a.n_tiles), dim3(kSelThreads), 0, stream, a);
     hipLaunchKernelGGL(select_scan_kernel, dim3(1), dim3(1024), 0, stream, a);
     if (a.n_tiles > 0)
         hipLaunchKernelGGL(select_scatter_kernel, dim3(a.n_tiles), dim3(kSelThreads), 0, stream, a);

@@ -181,6 +181,8 @@ __device__ __forceinline__ void store_probe(const RadianceArgs& a, int wg, int p
     const float al = a.alpha, bl = 1.0f - a.alpha;
     f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
     Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
+    // Duplicate requests of one probe: the scatter applies the last one.
+    if (a.claim) atomicMax(a.claim + ti, wg);
 }
 
 // K rays per lane.  tid is the thread's index within this world's launch
@@ -393,6 +395,9 @@ static hipError_t launch_lane(const RadianceArgs& a, int store, bool traced, int
 // Scatter of a staged subset update: probe i of the launch (request
 // i * size + shift) gets field_out[i] in field_in, in place.  Probes outside
 // the world were not updated and are skipped, as in the update itself.
+// Where several requests name one probe, the last of them (the highest i,
+// recorded in claim[] by the update) is applied and the claim word is reset
+// for the next update; the others leave the texel alone.
 template <int BYTES>
 __device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
     typedef typename std::conditional<BYTES == 16, uint4, uint2>::type T;
@@ -410,6 +415,10 @@ __device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
     }
     if ((unsigned)px >= (unsigned)w.wx || (unsigned)py >= (unsigned)w.wy || (unsigned)pz >= (unsigned)w.wz) return;
     const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
+    if (a.claim) {
+        if (a.claim[ti] != i) return;
+        a.claim[ti] = -1;
+    }
     gst((T*)a.field_in, ti, gld((const T*)a.field_out, (size_t)i));
 }
 
@@ -634,6 +643,7 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
             const float al = a.alpha, bl = 1.0f - a.alpha;
             f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
             Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
+            if (a.claim) atomicMax(a.claim + ti, wg);
         }
     }
     // The last workgroup to finish resets the batch counter for the next launch.

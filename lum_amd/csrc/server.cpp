@@ -83,6 +83,20 @@ struct lumx_server {
 
 static bool resident(const lumx_world* w) { return w->blocks != nullptr; }
 
+// Threads a request adds to its batched launch, bounded above: a radiance
+// request takes n_probes * rays / K threads padded to the workgroup (K >= 1,
+// workgroups of at most 1024), a glossy one w * h padded to 256.  A launch's
+// total must stay below 2^31.
+static const long long kLaunchThreads = (1ll << 31) - 1;
+static long long request_threads(const lumx_request& r, const lumx_world* w) {
+    if (r.kind == LUMX_REQ_RADIANCE) {
+        const long long n = r.requests_dev ? r.n_requests : w->n_blocks;
+        const long long probes = n / std::max(r.params.size, 1);
+        return (probes * r.params.rays + 1023) / 1024 * 1024;
+    }
+    return ((long long)r.width * r.height + 255) / 256 * 256;
+}
+
 // Make world i resident within the budget (caller holds the lock; `taken`
 // marks worlds already in the launch being built).  Returns false if the
 // budget cannot be met now.
@@ -193,6 +207,7 @@ static void worker_loop(lumx_server* s) {
             const int nw = (int)s->worlds.size();
             std::vector<char> taken(nw, 0), blocked(nw, 0);
             Flight f;
+            long long threads[2] = {0, 0};  // radiance, glossy threads of the launch being built
             for (auto it = s->pending.begin(); it != s->pending.end() && (int)f.reqs.size() < s->desc.max_batch;) {
                 Req* r = *it;
                 const int wi = r->spec.world;
@@ -201,6 +216,13 @@ static void worker_loop(lumx_server* s) {
                     // one radiance format per launch
                     ok = s->worlds[wi].w->desc.radiance_format == s->worlds[f.worlds[0]].w->desc.radiance_format;
                 }
+                const int kind = r->spec.kind == LUMX_REQ_RADIANCE ? 0 : 1;
+                const long long t = request_threads(r->spec, s->worlds[wi].w);
+                if (ok && threads[kind] + t > kLaunchThreads) {
+                    // The launch is full: this request and every later one of
+                    // its world wait for the next launch.
+                    ok = false;
+                }
                 if (ok) ok = ensure_resident(s, wi, taken);
                 if (!ok) {
                     blocked[wi] = 1;
@@ -208,6 +230,7 @@ static void worker_loop(lumx_server* s) {
                     continue;
                 }
                 taken[wi] = 1;
+                threads[kind] += t;
                 s->worlds[wi].busy++;
                 s->worlds[wi].last_use = ++s->use_clock;
                 f.reqs.push_back(r);
@@ -358,12 +381,18 @@ int lumx_server_submit(lumx_server_t s, const lumx_request* r, long long* ticket
             int rc = radiance_params_check(&r->params);
             if (rc) return rc;
             if (r->requests_dev && r->n_requests < 0) return fail(LUMX_EINVAL, "n_requests invalid");
+            const lumx_world* w = s->worlds[r->world].w;
+            if (r->requests_dev && r->n_requests / r->params.size > w->n_blocks)
+                return fail(LUMX_EINVAL, "%d requests at stride %d dispatch more probes than the world's %d",
+                            r->n_requests, r->params.size, w->n_blocks);
         } else if (!r->matnorm_dev || !r->depth_dev || !r->out_dev || r->width <= 0 || r->height <= 0 ||
                    r->width > 16384 || r->height > 16384) {
             return fail(LUMX_EINVAL, "glossy request buffers or size invalid");
         } else if (!(r->max_dist > 0.0f && r->max_dist <= 65536.0f) || r->max_steps < 0) {
             return fail(LUMX_EINVAL, "glossy max_dist must be in (0, 65536] and max_steps >= 0");
         }
+        if (request_threads(*r, s->worlds[r->world].w) > kLaunchThreads)
+            return fail(LUMX_EINVAL, "request too large for one launch (2^31 threads)");
         q->t_submit = Clock::now();
         *ticket = (long long)s->tickets.size();
         s->pending.push_back(q.get());

@@ -327,12 +327,15 @@ struct RadParams {
 };
 
 // One probe of radiance.comp:main.  'wg' is the workgroup (dispatch) index.
+// The new texel goes to out4[0..3]; returns false (nothing to store) for a
+// request outside the world.
 template <class T>
-void radiance_probe(const World& w, const Field<T>& fin, T* fout, const int32_t* reqs,
+bool radiance_probe(const World& w, const Field<T>& fin, T* out4, const int32_t* reqs,
                     const RadParams& p, const V3<float>* fib, int wg, int* ray_keys, int* ray_steps) {
     int req = wg * p.size + p.shift;
     const int32_t* q = reqs + 4 * (size_t)req;
     int px = q[0], py = q[1], pz = q[2];
+    if (px < 0 || py < 0 || pz < 0 || px >= w.wx || py >= w.wy || pz >= w.wz) return false;
     std::vector<V3<T>> lights(p.rays);
     for (int lane = 0; lane < p.rays; ++lane) {
         uint32_t gidx = (uint32_t)wg * (uint32_t)p.rays + (uint32_t)lane;
@@ -363,15 +366,18 @@ void radiance_probe(const World& w, const Field<T>& fin, T* fout, const int32_t*
     V3<T> o = {std::fmin(std::fmax(old[0], (T)0), (T)1), std::fmin(std::fmax(old[1], (T)0), (T)1),
                std::fmin(std::fmax(old[2], (T)0), (T)1)};
     T a = (T)p.alpha;
-    T* dst = fout + 4 * ((size_t)px + (size_t)fin.wx * ((size_t)py + (size_t)fin.wy * pz));
     // A NaN component stores as 0, as the reference's UNORM image stores it.
     auto unorm_nan = [](T v) { return v != v ? (T)0 : v; };
-    dst[0] = unorm_nan(o.x * ((T)1 - a) + mean.x * a);
-    dst[1] = unorm_nan(o.y * ((T)1 - a) + mean.y * a);
-    dst[2] = unorm_nan(o.z * ((T)1 - a) + mean.z * a);
-    dst[3] = (T)1;
+    out4[0] = unorm_nan(o.x * ((T)1 - a) + mean.x * a);
+    out4[1] = unorm_nan(o.y * ((T)1 - a) + mean.y * a);
+    out4[2] = unorm_nan(o.z * ((T)1 - a) + mean.z * a);
+    out4[3] = (T)1;
+    return true;
 }
 
+// Every dispatched probe reads field_in (Jacobi); the new texels are then
+// stored in dispatch order, so of several requests naming one probe the last
+// one's value remains (the reference's write order there is a race).
 template <class T>
 void radiance_update(const World& w, const T* field_in, T* field_out, const int32_t* reqs,
                      const RadParams& p, int threads, int* ray_keys, int* ray_steps) {
@@ -382,14 +388,23 @@ void radiance_update(const World& w, const T* field_in, T* field_out, const int3
     for (int i = 0; i < p.rays; ++i) fib[i] = fib_dir(i, p.rays);
     int n_wg = p.size > 0 ? p.n_req / p.size : 0;
     if (threads < 1) threads = 1;
+    std::vector<T> texels((size_t)n_wg * 4);
+    std::vector<uint8_t> stored((size_t)n_wg, 0);
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t) {
         pool.emplace_back([&, t]() {
             for (int wg = t; wg < n_wg; wg += threads)
-                radiance_probe<T>(w, fin, field_out, reqs, p, fib.data(), wg, ray_keys, ray_steps);
+                stored[wg] = radiance_probe<T>(w, fin, texels.data() + 4 * (size_t)wg, reqs, p, fib.data(), wg,
+                                               ray_keys, ray_steps);
         });
     }
     for (auto& th : pool) th.join();
+    for (int wg = 0; wg < n_wg; ++wg) {
+        if (!stored[wg]) continue;
+        const int32_t* q = reqs + 4 * ((size_t)wg * p.size + p.shift);
+        T* dst = field_out + 4 * ((size_t)q[0] + (size_t)w.wx * ((size_t)q[1] + (size_t)w.wy * q[2]));
+        for (int c = 0; c < 4; ++c) dst[c] = texels[4 * (size_t)wg + c];
+    }
 }
 
 }  // namespace
@@ -480,7 +495,12 @@ void orc_radiance_update_f64(const orc_world* w, const double* field_in, double*
 // workgroups.  fused != 0 evaluates the mat4 x vec4 rows as
 // fma(m2,z,fma(m1,y,m0*x)) + m3 (a contracted driver), else unfused
 // ((m0*x + m1*y) + m2*z) + m3.  Threads whose world coordinate is negative
-// on any axis write nothing (the reference races there).
+// alias as the shader's arithmetic makes them: ivec3() truncates toward zero,
+// iw / 16 truncates (OpSDiv) and iw % 16 takes the divisor's sign (OpSMod,
+// iw & 15), so -16 < iw < 0 lands in block 0.  Threads run in dispatch order
+// (x fastest, then y, then z), so where several threads write one voxel the
+// highest dispatch index is kept (the reference's write order there is a
+// race).
 // --------------------------------------------------------------------------
 void orc_map_model(int wx, int wy, int wz, const int16_t* blocks, uint8_t* palette,
                    int palette_slots, int static_count, const uint8_t* model, int mx, int my,
@@ -489,8 +509,6 @@ void orc_map_model(int wx, int wy, int wz, const int16_t* blocks, uint8_t* palet
     for (int gz = 0; gz < nz; ++gz)
     for (int gy = 0; gy < ny; ++gy)
     for (int gx = 0; gx < nx; ++gx) {
-        int ax = gx + shift[0], ay = gy + shift[1], az = gz + shift[2];
-        if (ax < 0 || ay < 0 || az < 0) continue;
         float wvx = ((float)gx + 0.5f) + (float)shift[0];
         float wvy = ((float)gy + 0.5f) + (float)shift[1];
         float wvz = ((float)gz + 0.5f) + (float)shift[2];
@@ -569,9 +587,11 @@ void orc_exec_copies(uint8_t* palette, int palette_slots, int static_count, int 
 }
 
 // update_radiance probe selection (internal_render.cpp:1142-1168): probes
-// whose clamped 3x3x3 neighbourhood sums to > 0, x-fastest, then the unique
-// special requests not already selected.  out: int32x4 per request.  Returns
-// the count.
+// whose clamped 3x3x3 neighbourhood sums to > 0, x-fastest, then every
+// special request whose probe the grid did not select, repeats included (the
+// reference sets `set` for grid probes only).  The list ends at one entry per
+// probe, the size of the reference's request buffer
+// (internal_render.cpp:198-200).  out: int32x4 per request.  Returns the count.
 int orc_select_probes(int wx, int wy, int wz, const int16_t* world, const int32_t* special,
                       int n_special, int32_t* out) {
     std::vector<uint8_t> set((size_t)wx * wy * wz, 0);
@@ -597,7 +617,7 @@ int orc_select_probes(int wx, int wy, int wz, const int16_t* world, const int32_
     for (int i = 0; i < n_special; ++i) {
         int x = special[4 * i], y = special[4 * i + 1], z = special[4 * i + 2];
         size_t idx = x + (size_t)wx * (y + (size_t)wy * z);
-        if (!set[idx]) {
+        if (!set[idx] && n < wx * wy * wz) {
             out[4 * n + 0] = x; out[4 * n + 1] = y; out[4 * n + 2] = z; out[4 * n + 3] = special[4 * i + 3];
             ++n;
         }

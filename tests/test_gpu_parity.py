@@ -181,6 +181,79 @@ def test_radiance_request_list_jacobi(gpu_ctx, tuning):
         w.close()
 
 
+@pytest.mark.parametrize("tuning", ["lane", "lane_occ", "lane_rpl4", "tiled"])
+@pytest.mark.parametrize("size_shift", [(1, 0), (2, 1)])
+def test_radiance_duplicate_requests_last_wins(gpu_ctx, tuning, size_shift):
+    """A list naming some probes several times (and some outside the world):
+    every entry is traced with its own RNG key, and of the entries naming one
+    probe the last one's texel is stored, as in the oracle."""
+    import torch
+    size = (12, 10, 6)
+    blocks, pal, mats, field = make_case(size, 77)
+    rng = np.random.default_rng(78)
+    wx, wy, wz = size
+    n = 300
+    flat = rng.integers(0, 40, size=n)  # 40 distinct probes, each named ~7 times
+    req = np.stack([flat % wx, (flat // wx) % wy, flat // (wx * wy), np.zeros(n, int)], 1).astype(np.int32)
+    req[5] = [wx, 0, 0, 0]       # outside: skipped
+    req[17] = [-1, 2, 1, 0]
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    w.set_tuning(**TUNINGS[tuning])
+    sz, sh = size_shift
+    try:
+        for frame in range(2):  # the claims reset between updates
+            w.radiance_update(torch.from_numpy(req).cuda(), radiance_params(rays=64, time_seed=3 + frame, size=sz,
+                                                                            shift=sh, alpha=0.3))
+        torch.cuda.synchronize()
+        got = w.read_radiance()
+        wd = orc.WorldData(blocks, full_palette(w, pal), mats)
+        ref = field
+        for frame in range(2):
+            ref = orc.radiance_update(wd, ref, requests=req, rays=64, time_seed=3 + frame, size=sz, shift=sh,
+                                      alpha=0.3)
+        assert np.array_equal(got, ref)
+        # the last entry of a repeated probe is the one stored: it differs
+        # from what the first entry alone would give
+        first = orc.radiance_update(wd, field, requests=req[:1], rays=64, time_seed=3, alpha=0.3)
+        assert not np.array_equal(first.reshape(-1, 4)[flat[0]],
+                                  orc.radiance_update(wd, field, requests=req, rays=64, time_seed=3,
+                                                      alpha=0.3).reshape(-1, 4)[flat[0]])
+    finally:
+        w.close()
+
+
+def test_request_list_longer_than_world_rejected(gpu_ctx):
+    """A list dispatching more probes than the world holds is refused before
+    any launch: the staged texels live in one texel per probe."""
+    import torch
+    from lum_amd._lib import LumxError, LUMX_EINVAL
+    size = (4, 4, 2)
+    blocks, pal, mats, field = make_case(size, 5)
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    try:
+        n_blocks = 32
+        req = torch.zeros((2 * (n_blocks + 1), 4), dtype=torch.int32, device="cuda")
+        with pytest.raises(LumxError) as e:
+            w.radiance_update(req[: n_blocks + 1], radiance_params(rays=32))
+        assert e.value.code == LUMX_EINVAL
+        with pytest.raises(LumxError):
+            w.radiance_update(req, radiance_params(rays=32, size=2, shift=1))
+        cnt = torch.tensor([1], dtype=torch.int32, device="cuda")
+        with pytest.raises(LumxError):
+            w.radiance_update_counted(req[: n_blocks + 1], cnt, radiance_params(rays=32))
+        # one dispatched probe per block at stride 2 is accepted (32 entries
+        # naming probe 0: the last one's texel is stored)
+        w.radiance_update(req[: 2 * n_blocks + 1], radiance_params(rays=32, size=2, alpha=0.5))
+        torch.cuda.synchronize()
+        got = w.read_radiance()
+        wd = orc.WorldData(blocks, full_palette(w, pal), mats)
+        ref = orc.radiance_update(wd, field, requests=req[: 2 * n_blocks + 1].cpu().numpy(), rays=32, size=2,
+                                  alpha=0.5)
+        assert np.array_equal(got, ref)
+    finally:
+        w.close()
+
+
 def test_radiance_frames_chain(gpu_ctx):
     """Three chained frames (time seed = frame) stay within tolerance."""
     import torch
@@ -265,6 +338,57 @@ def test_map_models_bit_exact(gpu_ctx):
         w.close()
 
 
+@pytest.mark.parametrize("device_blockify", [False, True])
+def test_map_models_crossing_origin_bit_exact(gpu_ctx, device_blockify):
+    """Models whose dispatch box reaches below 0 on one, two or three axes:
+    the threads at negative coordinates alias into block 0 as the shader's
+    truncating division and OpSMod make them (map.comp:40-66), the highest
+    dispatch index kept where threads collide; equal to the oracle bit for
+    bit, together with models placed inside the world."""
+    import torch
+    size = (12, 10, 6)
+    blocks, pal, mats, field = make_case(size, 23)
+    rng = np.random.default_rng(24)
+    models, quats, shifts = synth.make_models(rng, n=7, lo=6, hi=30, world_size=size)
+    # Pull models 0-4 across the origin on various axes (half-voxel offsets
+    # too), keep 5-6 inside.
+    pulls = [(-7.0, 3.0, 2.0), (4.0, -5.5, 1.0), (-3.25, -2.0, 0.5), (-9.0, -9.0, -4.0), (2.0, 2.0, -12.5)]
+    for i, d in enumerate(pulls):
+        shifts[i] = tuple(float(v) for v in d)
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    ms = [Model(gpu_ctx, m) for m in models]
+    try:
+        pls = [placement(q, s, m.size) for q, s, m in zip(quats, shifts, ms)]
+        assert sum(min(p.shift[0], p.shift[1], p.shift[2]) < 0 for p in pls) == 5
+        if device_blockify:
+            w.map_models_device(ms, pls)
+        else:
+            w.map_models_placed(ms, pls)
+        torch.cuda.synchronize()
+        counter = w.palette_counter()
+        got_pal = w.get_palette(0, counter)
+        cur = blocks.copy()
+        copy_src = np.zeros(w.palette_slots, np.int32)
+        c = pal.shape[0]
+        for p in pls:
+            c = orc.blockify(size, cur, pal.shape[0], c, list(p.aabb_min), list(p.aabb_max), copy_src)
+        assert c == counter
+        ref_pal = full_palette(w, pal)
+        orc.exec_copies(ref_pal, pal.shape[0], c, copy_src)
+        before = ref_pal.copy()
+        for m, p in zip(models, pls):
+            orc.map_model(size, cur, ref_pal, pal.shape[0], m, list(p.minv), list(p.shift)[:3], list(p.groups))
+        assert np.array_equal(w.get_blocks(blockified=True), cur)
+        assert np.array_equal(got_pal, ref_pal[:counter])
+        # block 0's slot was written through aliased coordinates
+        b0 = cur[0, 0, 0]
+        assert b0 >= pal.shape[0] and not np.array_equal(ref_pal[b0], before[b0])
+    finally:
+        for m in ms:
+            m.close()
+        w.close()
+
+
 def test_glossy_parity(gpu_ctx):
     import torch
     size = (48, 48, 16)
@@ -299,8 +423,10 @@ def test_select_probes_matches_oracle(gpu_ctx):
     try:
         special = np.array([[1, 1, 9, 0], [0, 0, 0, 0], [1, 1, 9, 0]], np.int32)
         got = w.select_probes(special)
-        ref = orc.select_probes(blocks, special[:2])
+        ref = orc.select_probes(blocks, special)
         assert np.array_equal(got, ref)
+        # The repeated special is appended twice (internal_render.cpp:1164-1168).
+        assert (got[:, :3] == [1, 1, 9]).all(1).sum() == 2
     finally:
         w.close()
 

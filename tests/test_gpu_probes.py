@@ -18,17 +18,12 @@ from tests.test_gpu_parity import make_case, gpu_world, TUNINGS
 pytestmark = pytest.mark.gpu
 
 
-def dedup_in_world(special, size):
-    """The specials the device keeps: inside the world, first occurrence."""
-    seen, keep = set(), []
-    for s in special:
-        x, y, z = int(s[0]), int(s[1]), int(s[2])
-        if not (0 <= x < size[0] and 0 <= y < size[1] and 0 <= z < size[2]):
-            continue
-        if (x, y, z) in seen:
-            continue
-        seen.add((x, y, z))
-        keep.append(s)
+def in_world(special, size):
+    """The specials the device considers: inside the world, repeats kept
+    (internal_render.cpp:1164-1168 appends every special whose probe the
+    grid did not select)."""
+    keep = [s for s in special
+            if 0 <= int(s[0]) < size[0] and 0 <= int(s[1]) < size[1] and 0 <= int(s[2]) < size[2]]
     return np.array(keep, np.int32).reshape(-1, 4)
 
 
@@ -72,14 +67,15 @@ def test_device_selection_matches_host_and_oracle(gpu_ctx, size, seed, kind):
     w = gpu_world(gpu_ctx, blocks, pal, mats, field)
     try:
         got, _, _ = device_select(w, special)
-        kept = dedup_in_world(special, size)
+        kept = in_world(special, size)
         ref = orc.select_probes(blocks, kept)
         assert np.array_equal(got, ref)
+        assert len(got) <= blocks.size  # one entry per probe at most
         host = w.select_probes(kept)
         assert np.array_equal(got, host)
         got0, _, _ = device_select(w, None)
         assert np.array_equal(got0, orc.select_probes(blocks, None))
-        # The claims are released: the same call again gives the same list.
+        # Nothing carries over between calls: the same call again gives the same list.
         again, _, _ = device_select(w, special)
         assert np.array_equal(again, got)
     finally:

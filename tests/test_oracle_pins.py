@@ -708,3 +708,263 @@ def test_fma_contraction_moves_few_rays():
     print(f"keys agree {agree:.6f}; texels within 1e-3 {(rel <= 1e-3).mean():.5f}; mean rel {rel.mean():.2e}")
     assert agree >= 0.9999
     assert (rel <= 1e-3).mean() >= 0.999
+
+
+# ---------------------------------------------------------------- radiance.comp main: ray setup
+def _decode_key(k):
+    v = int(k) - 1
+    return ((v & 1023) - 512, ((v >> 10) & 1023) - 512, ((v >> 20) & 1023) - 512)
+
+
+def _spec_rays(wg, probe, rays, t):
+    """The rays of dispatched probe `wg` as radiance.comp:362-381 sets them up,
+    stated here in float64 from the shader text: d0 = normalize(fib + 0.1 *
+    (2 * Random3D() - 1)), o = 16 * P + 8 + 6 * d0 + (2 * Random3D() - 1),
+    then d = d0 for even lanes and -d0 for odd ones (the flip comes after the
+    offset).  Random3D is keyed (gid.x + t, gid.y + t, t), gid.x = wg * R +
+    lane, and advances x between its two calls."""
+    out = []
+    c = np.array(probe, np.float64) * 16.0 + 8.0
+    for lane in range(rays):
+        g = wg * rays + lane
+        r0 = np.float64(orc.hash33([(g + t) % 2 ** 32, t, t]))
+        r1 = np.float64(orc.hash33([(g + t + 1) % 2 ** 32, t, t]))
+        d0 = np.float64(orc.fibonacci(lane, rays)) + 0.1 * (2.0 * r0 - 1.0)
+        d0 /= np.linalg.norm(d0)
+        o = c + 6.0 * d0 + (2.0 * r1 - 1.0)
+        out.append((o, d0 if lane % 2 == 0 else -d0))
+    return out
+
+
+def _ceiling_floor_world(wxy, wz, floor_top):
+    """Blocks with z < floor_top / 16 solid (every voxel material 1), air above."""
+    blocks = np.zeros((wz, wxy, wxy), np.int16)
+    blocks[: floor_top // 16] = 1
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    pal[1] = 1
+    return blocks, pal
+
+
+@pytest.mark.parametrize("rays,seed,wg", [(64, 0, 0), (64, 7, 5), (128, 3, 2), (32, 11, 0)])
+def test_ray_setup_hits_floor_where_the_shader_aims(rays, seed, wg):
+    """A probe above a solid floor (z < 16), in a world wide enough that no
+    ray leaves it sideways within the 128-voxel march.  Every ray starts in
+    air (the origin is within 6 + sqrt(3) of the probe centre at z = 24), so
+    a ray hits iff it points down and reaches z = 16 within the march, and its
+    hit voxel is the floor voxel where the ray crosses the plane z = 16
+    (closed-form plane intersection, not the march).  Which lanes hit, and
+    where, fixes the even/odd flip, its order after the origin offset, both
+    jitters and the RNG keying by dispatch index."""
+    blocks, pal = _ceiling_floor_world(17, 3, 16)
+    probe = (8, 8, 1)
+    req = np.zeros((wg + 1, 4), np.int32)
+    req[:] = [0, 0, 2, 0]
+    req[wg] = list(probe) + [0]
+    _, keys, _ = orc.radiance_update(world(blocks, pal), field_of(17, 17, 3, value=[0.1, 0.1, 0.1]),
+                                     requests=req, rays=rays, time_seed=seed, alpha=0.1, traced=True)
+    keys = keys.reshape(wg + 1, rays)[wg]
+    checked = hits = 0
+    for lane, (o, d) in enumerate(_spec_rays(wg, probe, rays, seed)):
+        assert np.linalg.norm(o - (np.array(probe) * 16 + 8)) <= 6 + math.sqrt(3) + 1e-9
+        assert o[2] > 16.0
+        t = (16.0 - o[2]) / d[2] if d[2] < 0 else np.inf
+        # The coarse march samples every 0.5 voxel up to fraction 128; skip
+        # rays within a sample of that limit or grazing a voxel boundary.
+        if abs(t - 128.0) < 1.0 or (d[2] < 0 and abs(d[2]) < 0.02):
+            continue
+        p = o + t * d if np.isfinite(t) else None
+        if p is not None and t < 128.0 and min(abs(p[0] - round(p[0])), abs(p[1] - round(p[1]))) < 0.02:
+            continue
+        checked += 1
+        if p is None or t >= 128.0:
+            assert keys[lane] == 0, lane
+            continue
+        hits += 1
+        assert keys[lane] != 0, lane
+        assert _decode_key(keys[lane]) == (int(math.floor(p[0])), int(math.floor(p[1])), 15), lane
+    assert checked >= rays * 3 // 4
+    assert rays // 4 <= hits <= 3 * rays // 4
+
+
+def test_ray_setup_lane_parity_decides_up_or_down():
+    """sphericalFibonacci's z = 1 - (2i + 1)/n is positive for the first half
+    of the lanes; the jitter (|0.1 * (2r - 1)| <= 0.1) cannot change the sign
+    where |z| > 0.24, and odd lanes are reversed.  So away from the equator a
+    lane hits the floor iff (i < n/2) == (i odd)."""
+    blocks, pal = _ceiling_floor_world(17, 3, 16)
+    for seed in range(3):
+        _, keys, _ = orc.radiance_update(world(blocks, pal), field_of(17, 17, 3, value=[0.1, 0.1, 0.1]),
+                                         requests=np.array([[8, 8, 1, 0]], np.int32), rays=64, time_seed=seed,
+                                         alpha=0.1, traced=True)
+        for i in range(64):
+            z = 1.0 - (2 * i + 1) / 64.0
+            if abs(z) <= 0.24:
+                continue
+            assert (keys[i] != 0) == ((i < 32) == (i % 2 == 1)), (seed, i)
+
+
+@pytest.mark.parametrize("rays,seed,wg", [(64, 1, 0), (32, 9, 3), (128, 5, 1)])
+def test_ray_setup_odd_lanes_come_back_through_the_probe(rays, seed, wg):
+    """Every origin lies 6 +- sqrt(3) from the probe centre along its
+    unflipped direction d0 (o = c + 6 d0 + jitter, |jitter| <= sqrt(3)), and
+    odd lanes are reversed after the offset, so an odd lane's line passes
+    within sqrt(3) of the centre, heading back through it, while an even
+    lane heads away.  A solid 4^3 core around the centre (half-width 2 >
+    sqrt(3); its corners 2 sqrt(3) < 6 - sqrt(3) from the centre, so no origin
+    lies in it) is therefore hit by every odd lane and by no even lane."""
+    blocks = np.zeros((3, 3, 3), np.int16)
+    blocks[1, 1, 1] = 1
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    pal[1, 6:10, 6:10, 6:10] = 4
+    req = np.zeros((wg + 1, 4), np.int32)
+    req[:] = [0, 0, 0, 0]
+    req[wg] = [1, 1, 1, 0]
+    _, keys, _ = orc.radiance_update(world(blocks, pal), field_of(3, 3, 3, value=[0.1, 0.2, 0.3]), requests=req,
+                                     rays=rays, time_seed=seed, alpha=0.1, traced=True)
+    keys = keys.reshape(wg + 1, rays)[wg]
+    for lane in range(rays):
+        if lane % 2 == 0:
+            assert keys[lane] == 0, lane
+        else:
+            assert keys[lane] != 0, lane
+            v = np.array(_decode_key(keys[lane]))
+            assert ((v >= 22) & (v < 26)).all(), (lane, v)
+
+
+# ---------------------------------------------------------------- radiance: repeated requests
+def test_repeated_request_stores_the_last_entry():
+    """Each dispatched entry traces with its own key (gid.x = wg * R + lane)
+    from the previous field; of the entries naming one probe the last one's
+    value is stored.  Entry 2 of [A, B, A] computes what entry 2 of [C, D, A]
+    computes, and differs from entry 0 of [A]."""
+    rng = np.random.default_rng(12)
+    from lum_amd import synth
+    size = (6, 5, 4)
+    blocks = synth.make_world(size, rng, 8)
+    pal = synth.make_palette(rng, 8)
+    mats = synth.make_materials(rng)
+    f = synth.make_field(size, rng)
+    w = orc.WorldData(blocks, pal, mats)
+    A, B, Cp, D = [2, 2, 1, 0], [3, 1, 2, 0], [0, 0, 0, 0], [5, 4, 3, 0]
+    idx = lambda q: (q[2], q[1], q[0])  # noqa: E731
+    aba = orc.radiance_update(w, f, requests=np.array([A, B, A], np.int32), rays=32, time_seed=4, alpha=0.5)
+    cda = orc.radiance_update(w, f, requests=np.array([Cp, D, A], np.int32), rays=32, time_seed=4, alpha=0.5)
+    a0 = orc.radiance_update(w, f, requests=np.array([A], np.int32), rays=32, time_seed=4, alpha=0.5)
+    assert np.array_equal(aba[idx(A)], cda[idx(A)])
+    assert not np.array_equal(aba[idx(A)], a0[idx(A)])
+    abb = orc.radiance_update(w, f, requests=np.array([A, B], np.int32), rays=32, time_seed=4, alpha=0.5)
+    assert np.array_equal(aba[idx(B)], abb[idx(B)])
+
+
+def test_select_probes_keeps_repeated_specials_and_caps_at_world_size():
+    """internal_render.cpp:1164-1168 marks only grid probes in `set`, so a
+    repeated special whose probe the grid did not select is appended every
+    time; the list holds one entry per probe at most (the request buffer,
+    internal_render.cpp:198-200)."""
+    b = np.zeros((2, 2, 3), np.int16)
+    sp = np.array([[0, 1, 1, 0], [2, 0, 0, 0], [0, 1, 1, 0], [0, 1, 1, 0]], np.int32)
+    got = orc.select_probes(b, sp)
+    assert [tuple(r[:3]) for r in got] == [(0, 1, 1), (2, 0, 0), (0, 1, 1), (0, 1, 1)]
+    many = np.array([[1, 1, 1, 0]] * 30, np.int32)
+    assert len(orc.select_probes(b, many)) == b.size
+    b[0, 0, 0] = 1  # the grid now takes (1, 1, 1) (and 7 other probes): no special is appended
+    grid = orc.select_probes(b, None)
+    assert len(grid) == 8 and (grid[:, :3] == [1, 1, 1]).all(1).any()
+    assert np.array_equal(orc.select_probes(b, many), grid)
+
+
+# ---------------------------------------------------------------- map.comp: negative coordinates
+def test_map_negative_coordinates_alias_into_block_zero():
+    """Identity rotation, translation (-3, 2, 2), model 8 x 4 x 4: the dispatch
+    covers g_x = -3..4.  With ivec3() truncating toward zero and % taking the
+    divisor's sign (OpSMod), g_x = -3, -2 (world -2.5, -1.5) land on voxels 14,
+    15 of block 0 with model x = 0, 1; g_x = -1 and 0 (world -0.5, 0.5) both
+    land on voxel 0 with model x = 2 and 3, the later thread (x = 3) winning
+    where its voxel is set; g_x = 1..4 give voxels 1..4 from x = 4..7."""
+    rng = np.random.default_rng(6)
+    model = rng.integers(1, 256, size=(4, 4, 8)).astype(np.uint8)  # z, y, x
+    model[1, 2, 3] = 0            # where x = 3 is empty, x = 2 (g_x = -1) remains
+    p = placement((1, 0, 0, 0), (-3.0, 2.0, 2.0), (8, 4, 4))
+    assert list(p.shift)[:3] == [-3, 2, 2] and list(p.groups) == [2, 1, 1]
+    blocks = np.zeros((1, 1, 1), np.int16)
+    blocks[0, 0, 0] = 1           # a dynamic slot (static count 1)
+    pal = map_into((1, 1, 1), blocks, 1, model, p, slots=2)
+    got = pal[1]                  # [z][y][x]
+    expect = np.zeros((16, 16, 16), np.uint8)
+    for gx in range(-3, 5):
+        vx = int(math.trunc(gx + 0.5)) & 15
+        mx = int(math.trunc(gx + 0.5 + 3.0))
+        for y in range(4):
+            for z in range(4):
+                v = model[z, y, mx]
+                if v:
+                    expect[2 + z, 2 + y, vx] = v
+    assert np.array_equal(got, expect)
+    assert got[3, 4, 0] == model[1, 2, 2] and got[2, 2, 0] == model[0, 0, 3]
+    assert got[2, 2, 14] == model[0, 0, 0] and got[2, 2, 15] == model[0, 0, 1]
+
+
+# ---------------------------------------------------------------- glossy: the hit branch
+_GLOSSY_CAM = dict(campos=(128.0, 128.0, 100.0), camdir=(0.3, 0.2, -0.9), horiz=(4.0, 0.0, 0.0),
+                   verti=(0.0, 4.0, 0.0), light=(0.5, 0.5, -0.9), frame=(8.0, 4.0), depth=20.0)
+
+
+def _ceiling_glossy_case(F, c_s, e_s, c_h, e_h, solid_all=False):
+    """A 256^3-voxel world with a solid ceiling (voxel z >= 160), surface
+    points below it at view depth 20 and a normal close to +z, so every
+    reflected ray goes up and meets the ceiling within the march."""
+    wb = 16
+    blocks = np.zeros((wb, wb, wb), np.int16)
+    if solid_all:
+        blocks[:] = 1
+    else:
+        blocks[10:] = 1
+    pal = np.zeros((2, 16, 16, 16), np.uint8)
+    pal[1] = 7
+    mats = np.zeros((256, 6), np.float32)
+    mats[3] = list(c_s) + [1.0, e_s, 0.1]   # surface (roughness <= 0.5: not masked)
+    mats[7] = list(c_h) + [1.0, e_h, 0.9]   # hit material
+    g = _GLOSSY_CAM
+    cd = np.array(g["camdir"], np.float64)
+    cd /= np.linalg.norm(cd)
+    ubo = np.array(list(g["campos"]) + list(cd) + list(g["horiz"]) + list(g["verti"]) + list(g["light"])
+                   + list(g["frame"]), np.float32)
+    mn = np.zeros((4, 8, 4), np.uint8)
+    mn[..., 0] = 3
+    mn[..., 1:3] = 128
+    mn[..., 3] = 255              # normal (1/255, 1/255, 1)
+    dp = np.full((4, 8), g["depth"] / 1000.0, np.float32)
+    f = field_of(wb, wb, wb, value=F)
+    return world(blocks, pal, mats), f, mn, dp, ubo
+
+
+@pytest.mark.parametrize("solid_all", [False, True])
+def test_glossy_hit_branch_closed_form(solid_all):
+    """glossy.frag:312-326,341-345: ProcessHit at the surface gives
+    L = c_s (e_s + F), R = c_s; a march hit then gives R *= c_h,
+    L += R (e_h + F_h).  With a uniform field F_h = F, so every pixel is
+    (c_s (e_s + F) + c_s c_h (e_h + F)) / 8.  Under a solid ceiling the hit
+    voxel is where the reflected ray crosses the plane z = 192; in a fully
+    solid world the first sample hits (refinement fallback: normal 0)."""
+    F = np.float32([0.3, 0.2, 0.1])
+    c_s, e_s = np.float32([0.5, 0.25, 1.0]), np.float32(0.5)
+    c_h, e_h = np.float32([0.6, 0.9, 0.3]), np.float32(2.0)
+    w, f, mn, dp, ubo = _ceiling_glossy_case(F, c_s, e_s, c_h, e_h, solid_all)
+    out, keys = orc.glossy(w, f, mn, dp, ubo, 128.0, 0)
+    expect = (c_s * (e_s + F) + c_s * c_h * (e_h + F)) / 8
+    assert np.allclose(out[..., :3], expect, rtol=1e-5, atol=1e-7)
+    assert (keys != 0).all()
+    if solid_all:
+        return
+    n = np.array([128, 128, 255], np.float64) / 255.0 * 2 - 1
+    d0 = np.array(ubo[3:6], np.float64)
+    for py in range(4):
+        for px in range(8):
+            cx = (px + 0.5) / 8 * 2 - 1
+            cy = (py + 0.5) / 4 * 2 - 1
+            o = ubo[0:3].astype(np.float64) + ubo[6:9] * cx + ubo[9:12] * cy + d0 * _GLOSSY_CAM["depth"]
+            d = d0 - 2 * np.dot(n, d0) * n
+            t = (160.0 - o[2]) / d[2]
+            p = o + t * d
+            assert 0 < t < 128
+            assert _decode_key(keys[py, px]) == (int(math.floor(p[0])), int(math.floor(p[1])), 160)
