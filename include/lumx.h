@@ -204,6 +204,20 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models,
  * LUMX_EOVERFLOW. */
 int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models,
                            const lumx_placement* placements, void* stream);
+/* lumx_map_models_device without any allocation, host or device.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch (model boxes, ballot words, tile counts) is one
+ * fixed 4 MiB device array per GPU, held until the process ends, and the
+ * model boxes reach it as a kernel argument.  Limits: at most 64 models, and
+ * model boxes covering at most about 16,000 x 2048 blocks in total (the
+ * scratch); beyond either it returns LUMX_EINVAL.  Calls that overlap on one
+ * GPU (any worlds, any streams) race on the array: order them on one stream
+ * or with events.  Inside a capture the world's origin must already be
+ * uploaded (any earlier map did it), else LUMX_EINVAL.  Results equal
+ * lumx_map_models_device bit for bit. */
+int lumx_map_models_device_noalloc(lumx_world_t w, int n, const lumx_model_t* models,
+                                   const lumx_placement* placements, void* stream);
 /* Palette slots allocated by the last map (the reference's paletteCounter). */
 int lumx_world_palette_counter(lumx_world_t w, int* out);
 

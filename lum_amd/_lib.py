@@ -126,6 +126,7 @@ _SIGS = {
                                  C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "lumx_batch_set_shape": (_I, [_P, _I, _I]),
     "lumx_map_models_device": (_I, [_P, _I, _P, _P, _P]),
+    "lumx_map_models_device_noalloc": (_I, [_P, _I, _P, _P, _P]),
     "lumx_vox_parse": (_I, [_P, C.c_size_t, _P, _P, C.c_size_t, _P, _P]),
     "lumx_diffuse": (_I, [_P, _P, _P, _I, _I, C.POINTER(Camera), C.POINTER(Lightmap), _P, _P]),
     "lumx_camera_default": (_I, [_I, _I, C.POINTER(Camera)]),

@@ -204,6 +204,14 @@ class World:
         ps = (Placement * max(n, 1))(*placements)
         check(self.lib.lumx_map_models_device(self.handle, n, hs, ps, _stream()))
 
+    def map_models_device_noalloc(self, models, placements):
+        """map_models_device with the library's fixed scratch: allocates
+        nothing (at most 64 models)."""
+        n = len(models)
+        hs = (C.c_void_p * max(n, 1))(*[m.handle.value for m in models])
+        ps = (Placement * max(n, 1))(*placements)
+        check(self.lib.lumx_map_models_device_noalloc(self.handle, n, hs, ps, _stream()))
+
     def radiance_update(self, requests=None, params=None, traced=False, **kw):
         """requests: torch int32 (n, 4) on the device, or None for every probe."""
         import torch

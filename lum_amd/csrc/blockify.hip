@@ -28,7 +28,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "lumx_internal.hpp"
+
+// Scratch of the no-allocation device map (lumx_map_models_device_noalloc):
+// model boxes and prefix, ballot words, tile counts.  Every GPU holds its own
+// copy until the process ends; host code reaches it only through the address
+// noalloc_scratch resolves, never through this name.
+static __device__ __align__(256) unsigned char scratch[lumx::kNoAllocScratchBytes];
 
 namespace lumx {
 
@@ -36,6 +44,14 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTile = 2048;
+
+__global__ __launch_bounds__(64) void blockify_desc_kernel(BlockifyDesc d, int4* boxes, int* first) {
+    const int i = threadIdx.x;
+    if (i >= d.n) return;
+    first[i] = d.first[i];
+    boxes[2 * i] = d.boxes[2 * i];
+    boxes[2 * i + 1] = d.boxes[2 * i + 1];
+}
 
 // Element t of the concatenated boxes: its model and block index.
 __device__ __forceinline__ void element(const BlockifyArgs& a, int t, int* m_out, unsigned* b_out) {
@@ -189,6 +205,30 @@ hipError_t launch_blockify(const BlockifyArgs& a, int copy_slots, hipStream_t st
     if (copy_slots > 0)
         hipLaunchKernelGGL(copy_bounded_kernel, dim3(copy_slots), dim3(256), 0, stream, a.palette, a.copy_src,
                            a.state, a.static_count);
+    return hipGetLastError();
+}
+
+hipError_t noalloc_scratch(int device, void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* ptr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[device]) {
+        hipError_t e = hipSetDevice(device);
+        if (e != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptr[device] = p;
+    }
+    *out = ptr[device];
+    return hipSuccess;
+}
+
+hipError_t launch_blockify_desc(const BlockifyDesc& d, int4* boxes, int* first, hipStream_t stream) {
+    if (d.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(blockify_desc_kernel, dim3(1), dim3(64), 0, stream, d, boxes, first);
     return hipGetLastError();
 }
 

@@ -238,6 +238,12 @@ int lumx_context_create(int device, lumx_context_t* out) {
         delete c;
         return fail(LUMX_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
+    e = noalloc_scratch(device, &c->noalloc_scratch);
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(LUMX_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
     // Direction tables of every power-of-two ray count up front, so that no
     // pass allocates while a graph is being captured.
     for (int r = 1; r <= 1024; r <<= 1) {
@@ -881,9 +887,9 @@ static int blockify(lumx_world* w, const lumx_placement& p) {
 static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
                          hipStream_t s) {
     const int st = w->desc.static_blocks;
-    std::vector<MapArgs> args(std::max(n, 1));
-    std::vector<int> layer(std::max(n, 1), 0);
-    std::vector<char> negative(std::max(n, 1), 0);
+    SmallBuf<MapArgs, kNoAllocModels> args(std::max(n, 1));
+    SmallBuf<int, kNoAllocModels> layer(std::max(n, 1), 0);
+    SmallBuf<char, kNoAllocModels> negative(std::max(n, 1), 0);
     if (w->origin_ids_dirty) {
         w->origin_has_dynamic = false;
         for (int16_t b : w->origin)
@@ -925,7 +931,7 @@ static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const
             if (overlap) layer[i] = std::max(layer[i], layer[j] + 1);
         }
     }
-    const int n_layers = n ? *std::max_element(layer.begin(), layer.begin() + n) + 1 : 0;
+    const int n_layers = n ? *std::max_element(&layer[0], &layer[0] + n) + 1 : 0;
     for (int l = 0; l < n_layers; ++l) {
         MapGroup g;
         g.n = 0;
@@ -980,9 +986,14 @@ static void block_box(const lumx_world* w, const lumx_placement& p, int mn[3], i
     }
 }
 
-int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
-                           void* stream) {
+// Device map: blockify, exec_copies and map.comp on the device.  With
+// `fixed` the scratch is the device's fixed array and the model boxes reach it
+// as a kernel argument, so nothing is allocated (host or device).
+static int map_device(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl, void* stream,
+                      bool fixed) {
     if (!w || n < 0 || (n > 0 && (!models || !pl))) return fail(LUMX_EINVAL, "bad argument");
+    if (fixed && n > kNoAllocModels)
+        return fail(LUMX_EINVAL, "no-allocation map: %d models (at most %d)", n, kNoAllocModels);
     for (int i = 0; i < n; ++i)
         if (!models[i] || models[i]->ctx != w->ctx) return fail(LUMX_EINVAL, "model %d invalid", i);
     REQUIRE_RESIDENT(w);
@@ -991,8 +1002,8 @@ int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, co
     const bool capturing = stream_capturing(s);
     const int st = w->desc.static_blocks, slots = w->desc.palette_slots;
     // Model boxes and their prefix over the concatenated element sequence.
-    std::vector<int> first(std::max(n, 1));
-    std::vector<int> box(8 * std::max(n, 1));  // per model: int4 min, int4 extent
+    SmallBuf<int, kNoAllocModels> first(std::max(n, 1), 0);
+    SmallBuf<int, 8 * kNoAllocModels> box(8 * std::max(n, 1), 0);  // per model: int4 min, int4 extent
     long long total = 0;
     for (int i = 0; i < n; ++i) {
         int mn[3], mx[3];
@@ -1013,7 +1024,14 @@ int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, co
     const size_t off_bits = off_box + align((size_t)std::max(n, 1) * 32);
     const size_t off_tiles = off_bits + align(tiles * (2048 / 8));
     const size_t need = off_tiles + align(std::max<size_t>(tiles, 1) * 4);
-    if (need > w->blockify_scratch_bytes) {
+    if (fixed && need > kNoAllocScratchBytes)
+        return fail(LUMX_EINVAL, "no-allocation map: model boxes cover %d blocks, scratch of %zu bytes holds "
+                                 "at most %zu", n_el, need, kNoAllocScratchBytes);
+    if (fixed && capturing && w->origin_dirty)
+        return fail(LUMX_EINVAL, "no-allocation map inside a capture: the origin world changed since its last "
+                                 "upload; map once outside the capture first");
+    if (fixed && !w->ctx->noalloc_scratch) return fail(LUMX_EDEVICE, "no-allocation scratch not resolved");
+    if (!fixed && need > w->blockify_scratch_bytes) {
         if (capturing)
             return fail(LUMX_EINVAL, "device map scratch too small inside a capture: run the same map once "
                                      "outside the capture first");
@@ -1029,8 +1047,28 @@ int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, co
     // map recorded into a graph gets its own, read on every replay.
     const size_t desc_bytes = off_bits;
     const size_t origin_bytes = w->origin_dirty ? (size_t)w->n_blocks * 2 : 0;
+    char* base = fixed ? (char*)w->ctx->noalloc_scratch : (char*)w->blockify_scratch;
     char* stage = nullptr;
-    if (capturing) {
+    if (fixed) {
+        // Boxes as a kernel argument; the origin world (when it changed)
+        // through the world's own pinned staging.
+        BlockifyDesc d{};
+        d.n = n;
+        for (int i = 0; i < n; ++i) {
+            d.first[i] = first[i];
+            d.boxes[2 * i] = make_int4(box[8 * i], box[8 * i + 1], box[8 * i + 2], 0);
+            d.boxes[2 * i + 1] = make_int4(box[8 * i + 4], box[8 * i + 5], box[8 * i + 6], 0);
+        }
+        if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));
+        HIP_TRY(launch_blockify_desc(d, (int4*)(base + off_box), (int*)base, s));
+        if (origin_bytes) {  // not capturing (checked above)
+            if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
+            w->stage_pending = false;
+            std::memcpy(w->stage_blocks, w->origin.data(), origin_bytes);
+            HIP_TRY(hipMemcpyAsync(w->origin_dev, w->stage_blocks, origin_bytes, hipMemcpyHostToDevice, s));
+            w->origin_dirty = false;
+        }
+    } else if (capturing) {
         void* p = nullptr;
         HIP_TRY(hipHostMalloc(&p, desc_bytes + origin_bytes + 16, hipHostMallocDefault));
         w->graph_stage.push_back(p);
@@ -1047,11 +1085,13 @@ int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, co
         }
         stage = (char*)w->stage_desc;
     }
-    std::memcpy(stage, first.data(), (size_t)std::max(n, 1) * 4);
-    std::memcpy(stage + off_box, box.data(), (size_t)std::max(n, 1) * 32);
-    if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));
-    HIP_TRY(hipMemcpyAsync(w->blockify_scratch, stage, desc_bytes, hipMemcpyHostToDevice, s));
-    if (origin_bytes) {
+    if (!fixed) {
+        std::memcpy(stage, &first[0], (size_t)std::max(n, 1) * 4);
+        std::memcpy(stage + off_box, &box[0], (size_t)std::max(n, 1) * 32);
+        if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));
+        HIP_TRY(hipMemcpyAsync(w->blockify_scratch, stage, desc_bytes, hipMemcpyHostToDevice, s));
+    }
+    if (!fixed && origin_bytes) {
         int16_t* so = capturing ? (int16_t*)(stage + desc_bytes) : w->stage_blocks;
         std::memcpy(so, w->origin.data(), origin_bytes);
         HIP_TRY(hipMemcpyAsync(w->origin_dev, so, origin_bytes, hipMemcpyHostToDevice, s));
@@ -1075,7 +1115,6 @@ int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, co
     a.n_models = n;
     a.n_elements = n_el;
     a.n_tiles = (int)tiles;
-    char* base = (char*)w->blockify_scratch;
     a.first = (const int*)base;
     a.boxes = (const int4*)(base + off_box);
     a.bits = (unsigned long long*)(base + off_bits);
@@ -1098,6 +1137,16 @@ int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, co
     HIP_TRY(hipEventRecord(w->stage_done, s));
     w->stage_pending = true;
     return LUMX_OK;
+}
+
+int lumx_map_models_device(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                           void* stream) {
+    return map_device(w, n, models, pl, stream, false);
+}
+
+int lumx_map_models_device_noalloc(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                                   void* stream) {
+    return map_device(w, n, models, pl, stream, true);
 }
 
 int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,

@@ -26,6 +26,28 @@ void clear_error();
                                 __FILE__, __LINE__);                                               \
     } while (0)
 
+// n elements of T: on the stack up to N, else on the heap.  The
+// no-allocation paths stay below N.
+template <class T, int N>
+struct SmallBuf {
+    T local[N];
+    std::vector<T> heap;
+    T* p;
+    explicit SmallBuf(int n, const T& v = T()) {
+        if (n <= N) {
+            for (int i = 0; i < N; ++i) local[i] = v;
+            p = local;
+        } else {
+            heap.assign((size_t)n, v);
+            p = heap.data();
+        }
+    }
+    SmallBuf(const SmallBuf&) = delete;
+    SmallBuf& operator=(const SmallBuf&) = delete;
+    T& operator[](int i) { return p[i]; }
+    const T& operator[](int i) const { return p[i]; }
+};
+
 // Two timing events destroyed on every return path.
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
@@ -45,6 +67,7 @@ struct lumx_context {
     int n_cu = 256;
     hipStream_t stream = nullptr;
     std::map<int, float4*> fib_tables;  // rays -> device table of per-lane directions
+    void* noalloc_scratch = nullptr;     // the device's fixed scratch array of the no-allocation map
     std::mutex mu;
 };
 

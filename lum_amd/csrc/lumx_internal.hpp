@@ -156,6 +156,23 @@ struct BlockifyArgs {
 size_t blockify_tiles(int n_elements);
 hipError_t launch_blockify(const BlockifyArgs& a, int copy_slots, hipStream_t stream);
 
+// No-allocation device map (blockify.hip): the model boxes, ballot words and
+// tile counts live in one fixed device array per GPU instead of the world's
+// scratch buffer, and the boxes arrive as a kernel argument instead of a
+// staged copy.
+constexpr int kNoAllocModels = 64;
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+struct BlockifyDesc {
+    int4 boxes[2 * kNoAllocModels];  // per model: first block, extent
+    int first[kNoAllocModels];       // per model: index of its first element
+    int n;
+};
+// Device address of the fixed array on `device`, resolved once per device
+// (the first call makes `device` current).
+hipError_t noalloc_scratch(int device, void** out);
+// Writes d's boxes and prefix to `boxes` and `first` on the device.
+hipError_t launch_blockify_desc(const BlockifyDesc& d, int4* boxes, int* first, hipStream_t stream);
+
 size_t select_tiles(int n_probes);
 size_t select_bits_bytes(int n_probes);
 int select_claim_idle();
