@@ -21,6 +21,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction
+# per SIMD every 4 cycles (16 lanes wide), at the 2.4 GHz peak engine clock.
+VALU_PEAK_GINST = 1024 * 2.4e9 / 4 / 1e9
 
 
 def measured_traffic(kernel):
@@ -34,6 +37,43 @@ def measured_traffic(kernel):
     except (OSError, ValueError):
         return None
     return None if rec is None else rec.get("traffic_bytes")
+
+
+def sq_record(kernel):
+    """SQ counters per launch of `kernel` from the committed profile
+    (profiles/sq.json, written from a rocprofv3 --pmc run of this bench on the
+    same launch), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "sq.json")) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def roofline_entry(kernel, kern_ms, bytes_per_launch, launch):
+    """Roofline of one launch of `kernel`, timed live (kern_ms).  The bound is
+    VALU issue when the SQ profile of this launch is recorded (instructions
+    per launch / time against the issue peak), else HBM.  The gathered-bytes
+    rate (algorithmic bytes, mostly served by the caches) and the measured HBM
+    rate (FETCH_SIZE + WRITE_SIZE per launch) are reported beside it."""
+    t = kern_ms / 1e3
+    gathered = bytes_per_launch / t / 1e9
+    traffic = measured_traffic(kernel)
+    hbm = None if traffic is None else traffic / t / 1e9
+    sq = sq_record(kernel)
+    if sq is not None:
+        achieved = sq["valu_insts"] / t / 1e9
+        entry = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_GINST,
+                 "unit": "G VALU wave-instructions/s", "frac": achieved / VALU_PEAK_GINST,
+                 "valu_insts_per_launch": sq["valu_insts"], "valu_insts_per_wave": sq.get("valu_per_wave")}
+    else:
+        achieved = hbm if hbm is not None else gathered
+        entry = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": achieved / HBM_PEAK_GBS}
+    entry.update({"traffic": traffic, "kernel": kernel, "kernel_ms": kern_ms, "bytes_per_launch": bytes_per_launch,
+                  "gathered_GBps": gathered, "gathered_frac_of_hbm_peak": gathered / HBM_PEAK_GBS,
+                  "hbm_GBps": hbm, "hbm_frac": None if hbm is None else hbm / HBM_PEAK_GBS, "launch": launch})
+    return entry
 
 
 def radiance_bytes(keys, steps, n_probes, texel_bytes, request_bytes=0):
@@ -323,11 +363,13 @@ REQUESTS_PER_STEP = 20
 
 def run_serving(args):
     """200 resident worlds (log-uniform 16x16x8..128x128x32 blocks, FP16
-    radiance).  A Poisson stream of 80% radiance updates (random 1-100% probe
-    subset, 32/64/128 rays) and 20% 640x360 reflection queries is served with
-    continuous batching and, as the plain baseline, one request per launch.
-    Reports saturated probe-rays/s of both and p50/p99 latency at a stated
-    rate (70% of the plain engine's saturated request rate)."""
+    radiance) under an HBM budget of half their device state.  A stream of
+    80% radiance updates (random 1-100% probe subset, 32/64/128 rays, no step
+    cap) and 20% 640x360 reflection queries is served with continuous
+    batching and, as the plain baseline, by the direct engine: one
+    single-world call per request, queued on one stream with no wait between
+    calls.  Reports saturated probe-rays/s of both and p50/p99 latency at a
+    stated rate (70% of the plain engine's saturated request rate)."""
     import torch
     from lum_amd import synth
     from lum_amd.api import (Context, World, Batch, Server, radiance_params, camera_default, radiance_request,
@@ -337,6 +379,7 @@ def run_serving(args):
     rng = np.random.default_rng(4242)
     ctx = Context(0)
     n_worlds = args.serving_worlds
+    max_steps = args.max_steps if args.max_steps is not None else 0
     sizes = synth.log_uniform_sizes(rng, n_worlds, (16, 16, 8), (128, 128, 32))
     worlds, hosts, perms, perm_host, gbufs, outs = [], [], [], [], [], []
     W, H = 640, 360
@@ -364,6 +407,12 @@ def run_serving(args):
         gbufs.append((torch.from_numpy(mn).cuda(), torch.from_numpy(dp).cuda(), camera_default(W, H)))
         outs.append(torch.zeros((H, W, 4), dtype=torch.float32, device="cuda"))
     torch.cuda.synchronize()
+    world_bytes = sum(w.resident()[1] for w in worlds)
+    if args.hbm_budget_gb > 0:
+        budget = int(args.hbm_budget_gb * (1 << 30))
+    else:
+        budget = int(args.hbm_budget_frac * world_bytes)
+    budget = max(budget, max(w.resident()[1] for w in worlds)) if budget > 0 else 0
 
     # request trace
     # A serving step is REQUESTS_PER_STEP requests of the trace: --steps K
@@ -386,7 +435,8 @@ def run_serving(args):
         reqs = []
         for i, (kind, wi, off, cnt, rays) in enumerate(trace):
             if kind == "r":
-                r = radiance_request(server.ids[wi], radiance_params(rays=rays, time_seed=i, alpha=0.1, max_steps=64))
+                r = radiance_request(server.ids[wi], radiance_params(rays=rays, time_seed=i, alpha=0.1,
+                                                                     max_steps=max_steps))
                 r.requests_dev = perms[wi].data_ptr() + 16 * off
                 r.n_requests = cnt
             else:
@@ -400,10 +450,9 @@ def run_serving(args):
 
     # Launch shape of the served radiance launches (workgroup size x rays per
     # lane), chosen by the batch autotuner over max_batch random resident
-    # worlds with rays from the request mix; the plain engine keeps the
-    # default shape (256 threads, one ray per thread).
-    # The trial launch is one batch of the mix: a random probe subset of 1-100%
-    # per world (slices of the same permutations the requests use).
+    # worlds with rays from the request mix.  The trial launch is one batch of
+    # the mix: a random probe subset of 1-100% per world (slices of the same
+    # permutations the requests use).
     pick = rng.choice(n_worlds, size=min(args.max_batch, n_worlds), replace=False)
     tune_params, tune_reqs = [], []
     for i in pick:
@@ -412,7 +461,7 @@ def run_serving(args):
         off = int(rng.integers(n))
         tune_reqs.append(perms[i][off:off + cnt])
         tune_params.append(radiance_params(rays=int(rng.choice([32, 64, 128])), time_seed=0, alpha=0.1,
-                                           max_steps=64))
+                                           max_steps=max_steps))
     if args.served_shape:
         served_block, served_rpl = (int(v) for v in args.served_shape.split(","))
     else:
@@ -421,10 +470,14 @@ def run_serving(args):
         tune_batch.close()
     torch.cuda.synchronize()
 
-    def serve(max_batch, streams, arrivals):
-        shape = (served_block, served_rpl) if max_batch > 1 else (0, 0)
-        srv = Server(ctx, worlds, hbm_budget=args.hbm_budget_gb * (1 << 30), max_batch=max_batch, streams=streams,
-                     block=shape[0], rays_per_lane=shape[1])
+    def serve(engine, arrivals):
+        if engine == "served":
+            srv = Server(ctx, worlds, hbm_budget=budget, max_batch=args.max_batch, streams=args.streams,
+                         block=served_block, rays_per_lane=served_rpl)
+        elif engine == "plain":
+            srv = Server(ctx, worlds, hbm_budget=budget, direct=True)
+        else:  # one launch at a time, each awaited before the next
+            srv = Server(ctx, worlds, hbm_budget=budget, max_batch=1, streams=1)
         lat, wall = srv.run_trace(build(srv)[:len(arrivals)], arrivals)
         st = srv.stats()
         srv.close()
@@ -432,14 +485,16 @@ def run_serving(args):
 
     burst = [0.0] * n_req
     if n_warm > 0:
-        serve(1, 1, burst[: min(n_warm, n_req)])  # warm-up
-    plain_lat_s, plain_wall_s, plain_st = serve(1, 1, burst)
-    served_lat_s, served_wall_s, served_st = serve(args.max_batch, args.streams, burst)
+        for engine in ("plain", "served"):
+            serve(engine, burst[: min(n_warm, n_req)])
+    plain_lat_s, plain_wall_s, plain_st = serve("plain", burst)
+    sync_lat_s, sync_wall_s, sync_st = serve("plain_sync", burst)
+    served_lat_s, served_wall_s, served_st = serve("served", burst)
     rate = 0.7 * n_req / plain_wall_s
     gaps = rng.exponential(1.0 / rate, size=n_req)
     arrivals = list(np.cumsum(gaps) - gaps[0])
-    plain_lat, plain_wall, _ = serve(1, 1, arrivals)
-    served_lat, served_wall, served_rate_st = serve(args.max_batch, args.streams, arrivals)
+    plain_lat, plain_wall, _ = serve("plain", arrivals)
+    served_lat, served_wall, served_rate_st = serve("served", arrivals)
 
     def pct(a, q):
         return float(np.percentile(np.asarray(a) * 1e3, q))
@@ -448,14 +503,17 @@ def run_serving(args):
     plain_value = probe_rays / plain_wall_s
 
     # Dominant kernel: one batched radiance launch over every world (all probes,
-    # 64 rays, FP16), timed with events on this stream; algorithmic bytes from a
-    # traced single-world update of each world with the same parameters.
+    # 64 rays, FP16, no step cap), timed with events on this stream;
+    # algorithmic bytes from a traced single-world update of each world with
+    # the same parameters.
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
+    for w in worlds:
+        w.restore()
     batch = Batch(ctx, worlds)
     batch.set_shape(served_block, served_rpl)
     kernel_name = "radiance_batch_kernel<1,false,false,%d>" % served_rpl
-    rp = [radiance_params(rays=64, time_seed=0, alpha=0.1, max_steps=64) for _ in worlds]
+    rp = [radiance_params(rays=64, time_seed=0, alpha=0.1, max_steps=max_steps) for _ in worlds]
     bytes_per_launch = 0
     for w in worlds:
         keys, steps = w.radiance_update(None, rp[0], traced=True)
@@ -471,8 +529,8 @@ def run_serving(args):
         b.record()
     torch.cuda.synchronize()
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    achieved = bytes_per_launch / (kern_ms / 1e3) / 1e9
     batch.close()
+    torch.cuda.set_stream(torch.cuda.default_stream())
 
     out = {
         "metric": "probe-rays/s and p50/p99 request latency, batched+served vs plain, 1xMI355X",
@@ -487,33 +545,38 @@ def run_serving(args):
         "vs_baseline": None,
         "dtype": "f32 (fp16 radiance storage)",
         "data": "synthetic (seeded worlds, palettes, materials, G-buffers, request trace)",
-        "config": {"workload": "serving: %d resident worlds log-uniform 16x16x8..128x128x32 blocks, FP16 radiance; "
-                               "%d requests, 80%% radiance (1-100%% probe subset, 32/64/128 rays, max 64 steps), "
-                               "20%% glossy 640x360 (max 128 steps)" % (n_worlds, n_req),
+        "config": {"workload": "serving: %d resident worlds log-uniform 16x16x8..128x128x32 blocks, FP16 radiance, "
+                               "HBM budget %.0f%% of their device state; %d requests, 80%% radiance (1-100%% probe "
+                               "subset, 32/64/128 rays, %s), 20%% glossy 640x360 (max 128 steps)"
+                               % (n_worlds, 100.0 * budget / world_bytes, n_req,
+                                  "max %d steps" % max_steps if max_steps else "no step cap"),
                    "requests": n_req, "requests_per_step": REQUESTS_PER_STEP,
                    "max_batch": args.max_batch, "streams": args.streams,
                    "served_shape": {"block": served_block, "rays_per_lane": served_rpl},
-                   "hbm_budget_gb": args.hbm_budget_gb, "probe_rays": probe_rays, "glossy_pixels": pixels},
+                   "hbm_budget_bytes": budget, "world_state_bytes": world_bytes,
+                   "probe_rays": probe_rays, "glossy_pixels": pixels},
         "plain_value": plain_value,
         "speedup_vs_plain": value / plain_value,
+        "plain_engine": "direct: one lumx_radiance_update / lumx_glossy call per request, queued on one stream "
+                        "with no wait between calls",
+        "swaps": {"served_evictions": served_st["evictions"], "served_restores": served_st["restores"],
+                  "plain_evictions": plain_st["evictions"], "plain_restores": plain_st["restores"]},
         "latency_ms": {"rate_req_per_s": rate,
                        "plain_p50": pct(plain_lat, 50), "plain_p99": pct(plain_lat, 99),
                        "served_p50": pct(served_lat, 50), "served_p99": pct(served_lat, 99),
                        "plain_wall_s": plain_wall, "served_wall_s": served_wall},
         "saturated": {"plain_wall_s": plain_wall_s, "served_wall_s": served_wall_s,
                       "plain_launches": plain_st["launches"], "served_launches": served_st["launches"],
-                      "plain_p50_ms": pct(plain_lat_s, 50), "served_p50_ms": pct(served_lat_s, 50)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kernel_name),
-                     "kernel": kernel_name, "kernel_ms": kern_ms,
-                     "bytes_per_launch": bytes_per_launch,
-                     "launch": "all %d worlds, every probe, 64 rays, FP16" % n_worlds},
+                      "plain_p50_ms": pct(plain_lat_s, 50), "served_p50_ms": pct(served_lat_s, 50),
+                      "plain_sync_value": probe_rays / sync_wall_s, "plain_sync_wall_s": sync_wall_s},
+        "roofline": roofline_entry(kernel_name, kern_ms, bytes_per_launch,
+                                   "all %d worlds, every probe, 64 rays, FP16" % n_worlds),
     }
     if args.copy_calibration_mb > 0:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration_mb)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_serving(trace, hosts, perm_host, sizes, seconds=args.cpu_seconds)
+        out["cpu_baseline"], out["error_vs_fp64"] = cpu_baseline_serving(ctx, trace, hosts, perm_host, sizes,
+                                                                         max_steps, seconds=args.cpu_seconds)
     for w in worlds:
         w.close()
     ctx.close()
@@ -532,9 +595,15 @@ def copy_calibration(mb):
     return n * 4
 
 
-def cpu_baseline_serving(trace, hosts, perm_host, sizes, seconds=10.0):
+def cpu_baseline_serving(ctx, trace, hosts, perm_host, sizes, max_steps, seconds=10.0):
     """The C++ oracle (all host threads, FP32 field from the same initial
-    values) on the trace's radiance requests in order, for about `seconds`."""
+    values) on the trace's radiance requests in order, for about `seconds`.
+
+    As the checker, the same leg states the served results' error: for the
+    first radiance request of up to 8 worlds (at most 4M probe-rays each) the
+    GPU's FP16 result of that request, computed on a fresh copy of its world
+    (served results equal these alone results bit for bit), against the FP64
+    oracle from the same initial field, over the requested probes' rgb."""
     import tests._oracle as orc
     threads = os.cpu_count() or 1
     fields = {}
@@ -548,16 +617,63 @@ def cpu_baseline_serving(trace, hosts, perm_host, sizes, seconds=10.0):
         wd = orc.WorldData(blocks, pal, mats)
         f = fields.get(wi, field)
         req = perm_host[wi][off:off + cnt]
-        fields[wi] = orc.radiance_update(wd, f, requests=req, rays=rays, time_seed=i, alpha=0.1, max_steps=64,
-                                         threads=threads)
+        fields[wi] = orc.radiance_update(wd, f, requests=req, rays=rays, time_seed=i, alpha=0.1,
+                                         max_steps=max_steps, threads=threads)
         done += cnt * rays
         n += 1
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "probe-rays/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} radiance requests of the serving trace ({done} probe-rays, {dt:.1f} s) by the "
-                      f"C++ oracle on {threads} threads"}
+    baseline = {"value": done / dt, "unit": "probe-rays/s", "cores": threads, "kind": "port",
+                "sample": f"first {n} radiance requests of the serving trace ({done} probe-rays, {dt:.1f} s) by the "
+                          f"C++ oracle on {threads} threads"}
+    return baseline, served_error_vs_fp64(ctx, trace, hosts, perm_host, sizes, max_steps, threads)
+
+
+def served_error_vs_fp64(ctx, trace, hosts, perm_host, sizes, max_steps, threads):
+    import torch
+    import tests._oracle as orc
+    from lum_amd.api import World, radiance_params
+    from lum_amd._lib import RADIANCE_F16
+    seen, sample = set(), []
+    for i, (kind, wi, off, cnt, rays) in enumerate(trace):
+        if kind == "r" and wi not in seen:
+            seen.add(wi)
+            if cnt * rays <= 4_000_000:
+                sample.append((i, wi, off, cnt, rays))
+        if len(sample) == 8:
+            break
+    rels = []
+    for i, wi, off, cnt, rays in sample:
+        blocks, pal, mats, field = hosts[wi]
+        req = perm_host[wi][off:off + cnt]
+        w = World(ctx, sizes[wi], static_blocks=64, palette_slots=64, radiance_format=RADIANCE_F16)
+        try:
+            w.set_blocks(blocks)
+            w.set_palette(pal)
+            w.set_materials(mats)
+            w.write_radiance(field)
+            w.upload_blocks()
+            w.radiance_update(torch.from_numpy(np.ascontiguousarray(req)).cuda(),
+                              radiance_params(rays=rays, time_seed=i, alpha=0.1, max_steps=max_steps))
+            torch.cuda.synchronize()
+            got = w.read_radiance()
+        finally:
+            w.close()
+        ref = orc.radiance_update(orc.WorldData(blocks, pal, mats), field.astype(np.float64), requests=req,
+                                  rays=rays, time_seed=i, alpha=0.1, max_steps=max_steps, threads=threads, fp64=True)
+        wx, wy = sizes[wi][0], sizes[wi][1]
+        idx = req[:, 0] + wx * (req[:, 1] + wy * req[:, 2])
+        g = got.reshape(-1, 4)[idx, :3].astype(np.float64)
+        r = ref.reshape(-1, 4)[idx, :3]
+        rels.append((np.abs(g - r) / np.maximum(np.abs(r), 1e-3)).ravel())
+    rel = np.concatenate(rels) if rels else np.zeros(1)
+    return {"storage": "fp16", "requests": len(sample), "texels": int(rel.size // 3),
+            "mean": float(rel.mean()), "p99": float(np.percentile(rel, 99)), "max": float(rel.max()),
+            "frac_within_1e-3": float((rel <= 1e-3).mean()),
+            "sample": "first radiance request of %d worlds of the trace (<= 4M probe-rays each), FP16 result on the "
+                      "GPU vs the FP64 oracle from the same initial field, rgb of the requested probes"
+                      % len(sample)}
 
 
 def run_chain(args):
@@ -743,7 +859,10 @@ def make_parser():
                     help="serving: timed requests (default 20 per step, 4000 at the default --steps)")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--streams", type=int, default=2)
-    ap.add_argument("--hbm-budget-gb", type=int, default=64)
+    ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
+                    help="serving: HBM budget in GiB (default: --hbm-budget-frac of the worlds' device state)")
+    ap.add_argument("--hbm-budget-frac", type=float, default=0.5,
+                    help="serving: HBM budget as a fraction of the worlds' device state (0: no budget)")
     ap.add_argument("--worlds", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)

@@ -349,7 +349,15 @@ int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
  * plus one batched glossy launch, and keeps up to `streams` launches in
  * flight while it admits the next.  Each request computes exactly what its
  * own lumx_radiance_update / lumx_glossy call would.  Worlds added to a
- * server belong to it until it is destroyed. */
+ * server belong to it until it is destroyed.
+ * With an HBM budget the server holds one device arena of the budget's size
+ * (or of all its worlds' bytes, if less) and every world lives there or in a
+ * pinned host snapshot: lumx_server_add_world moves a resident world to the
+ * host, a world larger than the budget is LUMX_EINVAL, and worlds are added
+ * before the first request.  Evictions and restores are copies on the
+ * server's own stream, never a host wait.  At destroy, worlds in the arena
+ * move into device blocks of their own; evicted ones stay on the host
+ * (lumx_world_restore brings them back). */
 #define LUMX_REQ_RADIANCE 0
 #define LUMX_REQ_GLOSSY 1
 
@@ -361,6 +369,12 @@ typedef struct {
     int streams;        /* launches in flight at once (>= 1) */
     int block;          /* radiance workgroup size: 0 (= 256), 256, 512 or 1024 */
     int rays_per_lane;  /* rays per thread of the radiance launches: 0 (= 1), 1, 2 or 4 */
+    int direct;         /* 1: the plain engine - every request is its own
+                         * lumx_radiance_update / lumx_glossy call, queued in
+                         * arrival order on one stream as soon as its world is
+                         * resident, up to 256 calls deep, with no wait between
+                         * them (max_batch, streams, block and rays_per_lane are
+                         * ignored) */
 } lumx_server_desc;
 
 typedef struct {

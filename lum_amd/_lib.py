@@ -65,7 +65,7 @@ class Lightmap(C.Structure):
 
 class ServerDesc(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("streams", C.c_int), ("block", C.c_int),
-                ("rays_per_lane", C.c_int)]
+                ("rays_per_lane", C.c_int), ("direct", C.c_int)]
 
 
 class Request(C.Structure):

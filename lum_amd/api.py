@@ -408,16 +408,23 @@ def glossy_request(world_id, matnorm, depth, cam, out, max_dist=128.0, max_steps
 class Server:
     """Continuous-batching request engine (lumx_server_*)."""
 
-    def __init__(self, ctx, worlds, hbm_budget=0, max_batch=32, streams=2, block=0, rays_per_lane=0):
+    def __init__(self, ctx, worlds, hbm_budget=0, max_batch=32, streams=2, block=0, rays_per_lane=0, direct=False):
+        """direct=True: the plain engine, one single-world call per request
+        queued on one stream with no wait between calls."""
         self.lib = ctx.lib
-        d = ServerDesc(int(hbm_budget), int(max_batch), int(streams), int(block), int(rays_per_lane))
+        d = ServerDesc(int(hbm_budget), int(max_batch), int(streams), int(block), int(rays_per_lane),
+                       1 if direct else 0)
         self.handle = C.c_void_p()
         check(self.lib.lumx_server_create(ctx.handle, C.byref(d), C.byref(self.handle)))
         self.ids = []
-        for w in worlds:
-            i = C.c_int()
-            check(self.lib.lumx_server_add_world(self.handle, w.handle, C.byref(i)))
-            self.ids.append(i.value)
+        try:
+            for w in worlds:
+                i = C.c_int()
+                check(self.lib.lumx_server_add_world(self.handle, w.handle, C.byref(i)))
+                self.ids.append(i.value)
+        except Exception:
+            self.close()
+            raise
 
     def submit(self, req):
         t = C.c_longlong()

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblumx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["radiance.hip", "map.hip", "glossy.hip", "probes.hip", "diffuse.hip", "blockify.hip", "host.cpp", "batch.cpp", "server.cpp", "clum.cpp", "vox.cpp"]
+SOURCES = ["radiance.hip", "map.hip", "glossy.hip", "probes.hip", "diffuse.hip", "blockify.hip", "residency.hip", "host.cpp", "batch.cpp", "server.cpp", "clum.cpp", "vox.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -19,6 +19,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "host_internal.hpp"
@@ -278,77 +279,138 @@ int lumx_context_sync(lumx_context_t c) {
 void* lumx_context_stream(lumx_context_t c) { return c ? (void*)c->stream : nullptr; }
 
 // ---- world ----
-// Device state of a world: blocks, palette slots, materials, copy list,
-// two radiance fields, occupancy bits, work counter (all zeroed).
+// Device state of a world, carved from one block, each part 256-byte
+// aligned, in this order:
+//   blocks | copy list | materials | palette slots | field[0] | field[1] |
+//   occupancy bits, claims, counters, origin world
+// What a world keeps while evicted is its current field and the range from
+// the blocks to the last palette slot in use (its "static" state: serving
+// only changes fields, so a server copies that range out once).
+struct WorldLayout {
+    size_t blocks, copy_src, mats, palette, field0, field1, empty_bits, pal_bits, batch_counter, sel_bits,
+        sel_tiles, sel_claim, upd_claim, origin, blockify_state, total;
+};
+
+static WorldLayout world_layout(const lumx_world* w) {
+    const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        const size_t at = off;
+        off += bytes;
+        return at;
+    };
+    WorldLayout L;
+    L.blocks = take(nb * sizeof(int16_t));
+    L.copy_src = take(slots * sizeof(int32_t));
+    L.mats = take(256 * 6 * sizeof(float));
+    L.palette = take(slots * 4096);
+    L.field0 = take(nb * w->texel_bytes);
+    L.field1 = take(nb * w->texel_bytes);
+    L.empty_bits = take(((nb + 31) / 32) * sizeof(uint32_t));
+    L.pal_bits = take(slots * 128 * sizeof(uint32_t));
+    L.batch_counter = take(2 * sizeof(unsigned int));
+    L.sel_bits = take(lumx::select_bits_bytes(w->n_blocks));
+    L.sel_tiles = take(lumx::select_tiles(w->n_blocks) * sizeof(int));
+    L.sel_claim = take(nb * sizeof(int));
+    L.upd_claim = take(nb * sizeof(int));
+    L.origin = take(nb * sizeof(int16_t));
+    L.blockify_state = take(4 * sizeof(int));
+    L.total = (off + 4095) & ~(size_t)4095;
+    return L;
+}
+
+static void world_carve(lumx_world* w, char* base) {
+    const WorldLayout L = world_layout(w);
+    w->blocks = (int16_t*)(base + L.blocks);
+    w->copy_src_dev = (int32_t*)(base + L.copy_src);
+    w->mats = (float*)(base + L.mats);
+    w->palette = (uint8_t*)(base + L.palette);
+    w->field[0] = base + L.field0;
+    w->field[1] = base + L.field1;
+    w->empty_bits = (uint32_t*)(base + L.empty_bits);
+    w->pal_bits = (uint32_t*)(base + L.pal_bits);
+    w->batch_counter = (unsigned int*)(base + L.batch_counter);
+    w->sel_bits = (unsigned long long*)(base + L.sel_bits);
+    w->sel_tiles = (int*)(base + L.sel_tiles);
+    w->sel_claim = (int*)(base + L.sel_claim);
+    w->upd_claim = (int*)(base + L.upd_claim);
+    w->origin_dev = (int16_t*)(base + L.origin);
+    w->blockify_state = (int*)(base + L.blockify_state);
+}
+
+// The host snapshot: the static range (blocks .. palette slot `used`) at
+// offset 0, the current field after it.
+static size_t snap_static_bytes(const lumx_world* w, int used) {
+    return world_layout(w).palette + (size_t)used * 4096;
+}
+static size_t snap_field_offset(const lumx_world* w, int used) {
+    return (snap_static_bytes(w, used) + 255) & ~(size_t)255;
+}
+
+extern "C++" {
+size_t lumx::world_bytes(const lumx_world* w) { return world_layout(w).total; }
+
+void lumx::world_rebase(lumx_world* w, void* block, bool own) {
+    w->dev_mem = block;
+    w->dev_own = own;
+    world_carve(w, (char*)block);
+}
+}  // extern "C++"
+
+// Idle values of everything a world does not keep while evicted: the
+// other field and the palette slots above `used` zeroed, claims idle,
+// counters zero (the map counter `counter`), occupancy bits rebuilt from the
+// blocks and the palette (one kernel on s).
+static hipError_t world_init_rest(lumx_world* w, int used, hipStream_t s) {
+    lumx::AttachArgs a{};
+    a.field_other = (uint2*)w->field[w->cur ^ 1];
+    a.n_field_words = (long long)w->n_blocks * (long long)(w->texel_bytes / 8);
+    a.palette = w->palette;
+    a.used = used;
+    a.slots = w->desc.palette_slots;
+    a.blocks = w->blocks;
+    a.n_blocks = w->n_blocks;
+    a.empty_bits = w->empty_bits;
+    a.pal_bits = w->pal_bits;
+    a.sel_claim = w->sel_claim;
+    a.claim_idle = lumx::select_claim_idle();
+    a.upd_claim = w->upd_claim;
+    a.batch_counter = w->batch_counter;
+    a.blockify_state = w->blockify_state;
+    a.counter = w->counter;
+    w->origin_dirty = true;
+    return lumx::launch_world_attach(a, s);
+}
+
+// A world's own device block, every part zeroed (claims and counters idle).
 static hipError_t world_alloc_device(lumx_world* w) {
     const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
-    hipError_t e = hipSuccess;
-    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
-    chk(hipMalloc(&w->blocks, nb * sizeof(int16_t)));
-    chk(hipMalloc(&w->palette, slots * 4096));
-    chk(hipMalloc(&w->mats, 256 * 6 * sizeof(float)));
-    chk(hipMalloc(&w->copy_src_dev, slots * sizeof(int32_t)));
-    chk(hipMalloc(&w->field[0], nb * w->texel_bytes));
-    chk(hipMalloc(&w->field[1], nb * w->texel_bytes));
-    chk(hipMalloc(&w->empty_bits, ((nb + 31) / 32) * sizeof(uint32_t)));
-    chk(hipMalloc(&w->pal_bits, slots * 128 * sizeof(uint32_t)));
-    chk(hipMalloc(&w->batch_counter, 2 * sizeof(unsigned int)));
-    chk(hipMalloc(&w->sel_bits, lumx::select_bits_bytes(w->n_blocks)));
-    chk(hipMalloc(&w->sel_tiles, lumx::select_tiles(w->n_blocks) * sizeof(int)));
-    chk(hipMalloc(&w->sel_claim, nb * sizeof(int)));
-    chk(hipMalloc(&w->upd_claim, nb * sizeof(int)));
-    chk(hipMalloc(&w->origin_dev, nb * sizeof(int16_t)));
-    chk(hipMalloc(&w->blockify_state, 4 * sizeof(int)));
+    void* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, w->device_bytes);
     if (e != hipSuccess) return e;
-    chk(hipMemset(w->blockify_state, 0, 4 * sizeof(int)));
-    w->origin_dirty = true;
-    static_assert(sizeof(int) == 4, "claim words");
-    chk(hipMemset(w->sel_claim, lumx::select_claim_idle() & 0xff, nb * sizeof(int)));
-    chk(hipMemset(w->upd_claim, 0xff, nb * sizeof(int)));  // -1: no request
-    chk(hipMemset(w->empty_bits, 0xff, ((nb + 31) / 32) * sizeof(uint32_t)));
-    chk(hipMemset(w->pal_bits, 0, slots * 128 * sizeof(uint32_t)));
-    chk(hipMemset(w->batch_counter, 0, 2 * sizeof(unsigned int)));
-    chk(hipMemset(w->blocks, 0, nb * sizeof(int16_t)));
-    chk(hipMemset(w->palette, 0, slots * 4096));
-    chk(hipMemset(w->mats, 0, 256 * 6 * sizeof(float)));
-    chk(hipMemset(w->copy_src_dev, 0, slots * sizeof(int32_t)));
-    chk(hipMemset(w->field[0], 0, nb * w->texel_bytes));
-    chk(hipMemset(w->field[1], 0, nb * w->texel_bytes));
+    w->dev_mem = mem;
+    w->dev_own = true;
+    world_carve(w, (char*)mem);
+    auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    chk(hipMemsetAsync(w->blocks, 0, nb * sizeof(int16_t), nullptr));
+    chk(hipMemsetAsync(w->mats, 0, 256 * 6 * sizeof(float), nullptr));
+    chk(hipMemsetAsync(w->copy_src_dev, 0, slots * sizeof(int32_t), nullptr));
+    chk(hipMemsetAsync(w->field[w->cur], 0, nb * w->texel_bytes, nullptr));
+    chk(world_init_rest(w, 0, nullptr));
+    chk(hipMemsetD32Async((hipDeviceptr_t)w->blockify_state, 0, 1, nullptr));
+    chk(hipStreamSynchronize(nullptr));
     return e;
 }
 
-static size_t world_device_bytes(const lumx_world* w) {
-    const size_t nb = (size_t)w->n_blocks, slots = (size_t)w->desc.palette_slots;
-    return nb * 2 + slots * 4096 + 256 * 6 * 4 + slots * 4 + 2 * nb * w->texel_bytes + ((nb + 31) / 32) * 4 +
-           slots * 128 * 4 + 8 + lumx::select_bits_bytes(w->n_blocks) +
-           lumx::select_tiles(w->n_blocks) * sizeof(int) + 2 * nb * sizeof(int) + nb * 2 + 16;
-}
-
+// Forgets the device block (freeing it when it is the world's own).
 static void world_free_device(lumx_world* w) {
-    hipFree(w->blocks);
-    hipFree(w->palette);
-    hipFree(w->mats);
-    hipFree(w->copy_src_dev);
-    hipFree(w->field[0]);
-    hipFree(w->field[1]);
-    hipFree(w->empty_bits);
-    hipFree(w->pal_bits);
-    hipFree(w->batch_counter);
-    hipFree(w->sel_bits);
-    hipFree(w->sel_tiles);
-    hipFree(w->sel_claim);
-    hipFree(w->upd_claim);
-    w->upd_claim = nullptr;
-    hipFree(w->origin_dev);
-    hipFree(w->blockify_state);
+    if (w->dev_own && w->dev_mem) hipFree(w->dev_mem);
+    w->dev_mem = nullptr;
+    w->dev_own = false;
     hipFree(w->blockify_scratch);
-    w->origin_dev = nullptr;
-    w->blockify_state = nullptr;
     w->blockify_scratch = nullptr;
     w->blockify_scratch_bytes = 0;
-    w->sel_bits = nullptr;
-    w->sel_tiles = nullptr;
-    w->sel_claim = nullptr;
     w->blocks = nullptr;
     w->palette = nullptr;
     w->mats = nullptr;
@@ -357,10 +419,73 @@ static void world_free_device(lumx_world* w) {
     w->empty_bits = nullptr;
     w->pal_bits = nullptr;
     w->batch_counter = nullptr;
+    w->sel_bits = nullptr;
+    w->sel_tiles = nullptr;
+    w->sel_claim = nullptr;
+    w->upd_claim = nullptr;
+    w->origin_dev = nullptr;
+    w->blockify_state = nullptr;
 }
+
+extern "C++" {
+int lumx::world_snapshot_reserve(lumx_world* w, int used, hipStream_t s) {
+    const size_t need = snap_field_offset(w, used) + (size_t)w->n_blocks * w->texel_bytes;
+    if (need <= w->snap_bytes) return LUMX_OK;
+    if (w->snap_host) {
+        // The previous snapshot may still be read or written by copies on s.
+        HIP_TRY(hipStreamSynchronize(s));
+        hipHostFree(w->snap_host);
+        w->snap_host = nullptr;
+        w->snap_bytes = 0;
+        w->snap_static_valid = false;
+    }
+    HIP_TRY(hipHostMalloc(&w->snap_host, need, hipHostMallocDefault));
+    w->snap_bytes = need;
+    return LUMX_OK;
+}
+
+int lumx::world_snapshot_async(lumx_world* w, hipStream_t s, bool field_only) {
+    int rc = sync_host_state(w);
+    if (rc) return rc;
+    if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
+    w->stage_pending = false;
+    const int used = std::max(w->counter, w->prev_counter);
+    if (field_only && !(w->snap_static_valid && used == w->snap_used))
+        return fail(LUMX_EINVAL, "no snapshot of the world's static state to keep");
+    rc = world_snapshot_reserve(w, used, s);
+    if (rc) return rc;
+    // Copies from the device on s, so the snapshot is written in stream
+    // order after whatever copy last read it.
+    char* h = (char*)w->snap_host;
+    if (!field_only || !w->snap_static_valid)
+        HIP_TRY(hipMemcpyAsync(h, w->dev_mem, snap_static_bytes(w, used), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h + snap_field_offset(w, used), w->field[w->cur], (size_t)w->n_blocks * w->texel_bytes,
+                           hipMemcpyDeviceToHost, s));
+    w->snap_used = used;
+    w->snap_static_valid = true;
+    if (w->dev_own) HIP_TRY(hipStreamSynchronize(s));  // the block is freed below
+    world_free_device(w);
+    return LUMX_OK;
+}
+
+int lumx::world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s) {
+    if (!w->snap_host) return fail(LUMX_EINVAL, "world has no snapshot to restore");
+    w->dev_mem = block;
+    w->dev_own = own;
+    world_carve(w, (char*)block);
+    const int used = w->snap_used;
+    const char* h = (const char*)w->snap_host;
+    HIP_TRY(hipMemcpyAsync(block, h, snap_static_bytes(w, used), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(w->field[w->cur], h + snap_field_offset(w, used), (size_t)w->n_blocks * w->texel_bytes,
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(world_init_rest(w, used, s));
+    return LUMX_OK;
+}
+}  // extern "C++"
 
 static void world_free(lumx_world* w) {
     world_free_device(w);
+    if (w->snap_host) hipHostFree(w->snap_host);
     if (w->stage_blocks) hipHostFree(w->stage_blocks);
     if (w->stage_copy_src) hipHostFree(w->stage_copy_src);
     if (w->stage_desc) hipHostFree(w->stage_desc);
@@ -401,7 +526,7 @@ int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* 
     w->counter = d->static_blocks;
     w->prev_counter = d->static_blocks;
     w->tune = default_tuning();
-    w->device_bytes = world_device_bytes(w);
+    w->device_bytes = lumx::world_bytes(w);
     hipError_t e = world_alloc_device(w);
     auto chk = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     chk(hipHostMalloc((void**)&w->stage_blocks, nb * sizeof(int16_t), hipHostMallocDefault));
@@ -701,50 +826,26 @@ int lumx_world_evict(lumx_world_t w) {
     if (!w) return fail(LUMX_EINVAL, "null world");
     if (!w->blocks) return LUMX_OK;
     HIP_TRY(hipSetDevice(w->ctx->device));
-    int rc = sync_host_state(w);
-    if (rc) return rc;
-    if (w->stage_pending) HIP_TRY(hipEventSynchronize(w->stage_done));
-    w->stage_pending = false;
-    const size_t nb = (size_t)w->n_blocks;
-    const size_t used = (size_t)std::max(w->counter, w->prev_counter);
-    w->snap_field.resize(nb * w->texel_bytes);
-    w->snap_palette.resize(used * 4096);
-    w->snap_mats.resize(256 * 6);
-    // Blocking copies on the null stream: the world's own work has completed
-    // (the caller evicts idle worlds only).
-    HIP_TRY(hipMemcpy(w->snap_field.data(), w->field[w->cur], nb * w->texel_bytes, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(w->snap_palette.data(), w->palette, used * 4096, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(w->snap_mats.data(), w->mats, 256 * 6 * sizeof(float), hipMemcpyDeviceToHost));
-    world_free_device(w);
-    return LUMX_OK;
+    // Work on the world may be queued on any stream.
+    HIP_TRY(hipDeviceSynchronize());
+    return world_snapshot_async(w, nullptr, false);
 }
 
 int lumx_world_restore(lumx_world_t w) {
     if (!w) return fail(LUMX_EINVAL, "null world");
     if (w->blocks) return LUMX_OK;
     HIP_TRY(hipSetDevice(w->ctx->device));
-    hipError_t e = world_alloc_device(w);
-    if (e != hipSuccess) {
-        world_free_device(w);
+    void* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, w->device_bytes);
+    if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? LUMX_ENOMEM : LUMX_EDEVICE, "restore: %s", hipGetErrorString(e));
+    int rc = world_attach_async(w, mem, true, nullptr);
+    if (rc == LUMX_OK) {
+        hipError_t se = hipStreamSynchronize(nullptr);
+        if (se != hipSuccess) rc = fail(LUMX_EDEVICE, "restore: %s", hipGetErrorString(se));
     }
-    const size_t nb = (size_t)w->n_blocks;
-    const int used = (int)(w->snap_palette.size() / 4096);
-    HIP_TRY(hipMemcpy(w->blocks, w->current.data(), nb * 2, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(w->palette, w->snap_palette.data(), w->snap_palette.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(w->mats, w->snap_mats.data(), 256 * 6 * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(w->field[w->cur], w->snap_field.data(), nb * w->texel_bytes, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(w->copy_src_dev, w->copy_src.data(), (size_t)w->desc.palette_slots * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(w->blockify_state, &w->counter, sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(launch_block_bits(w->blocks, w->n_blocks, w->empty_bits, nullptr));
-    HIP_TRY(launch_palette_bits(w->palette, 0, used, w->pal_bits, nullptr));
-    HIP_TRY(hipStreamSynchronize(nullptr));
-    w->snap_field.clear();
-    w->snap_field.shrink_to_fit();
-    w->snap_palette.clear();
-    w->snap_palette.shrink_to_fit();
-    w->snap_mats.clear();
-    return LUMX_OK;
+    if (rc) world_free_device(w);
+    return rc;
 }
 
 int lumx_world_resident(lumx_world_t w, int* resident, size_t* device_bytes) {

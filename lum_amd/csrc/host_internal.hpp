@@ -123,15 +123,40 @@ struct lumx_world {
     hipEvent_t stage_done = nullptr;    // recorded after the staging uploads of the last map
     bool stage_pending = false;
     std::vector<void*> graph_stage;     // pinned staging owned by maps recorded into graphs
-    // Host snapshot of the device state while evicted (blocks live in `current`).
-    std::vector<uint8_t> snap_field, snap_palette;
-    std::vector<float> snap_mats;
+    // Device block holding every device part of the world (world_carve), and
+    // whether the world allocated it (else a server arena lent it).
+    void* dev_mem = nullptr;
+    bool dev_own = false;
+    // Pinned host snapshot of the device state while evicted.
+    void* snap_host = nullptr;
+    size_t snap_bytes = 0;
+    int snap_used = 0;                  // palette slots in the snapshot
+    bool snap_static_valid = false;     // the snapshot holds the static state (blocks .. palette)
     int32_t* stage_copy_src = nullptr;  // pinned host staging of the copy list
     lumx_radiance_tuning tune{};
-    size_t device_bytes = 0;            // HBM held by this world
+    size_t device_bytes = 0;            // HBM held by this world (its device block)
 };
 
 namespace lumx {
+
+// Bytes of a world's device block.
+size_t world_bytes(const lumx_world* w);
+// Residency in steps on a stream (the server's copy stream, or the null
+// stream for lumx_world_evict / lumx_world_restore):
+//   world_snapshot_reserve  pinned snapshot space for `used` palette slots;
+//   world_snapshot_async    device state -> snapshot on s (only the field
+//                           when `field_only`: the caller knows the static
+//                           state has not changed since the last snapshot),
+//                           then the world forgets its block (freed after a
+//                           sync of s when it is its own; a lent block is the
+//                           lender's again);
+//   world_attach_async      the world takes `block` (its own when `own`) and
+//                           fills it from the snapshot on s.
+int world_snapshot_reserve(lumx_world* w, int used, hipStream_t s);
+int world_snapshot_async(lumx_world* w, hipStream_t s, bool field_only);
+int world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s);
+// The world's device parts now live in `block` (a copy of its old block).
+void world_rebase(lumx_world* w, void* block, bool own);
 
 // Validation shared by single-world and batched radiance updates.
 int radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p);

@@ -173,6 +173,26 @@ hipError_t noalloc_scratch(int device, void** out);
 // Writes d's boxes and prefix to `boxes` and `first` on the device.
 hipError_t launch_blockify_desc(const BlockifyDesc& d, int4* boxes, int* first, hipStream_t stream);
 
+// World attach (residency.hip): the parts of a world's device block that
+// are not kept while it is evicted.
+struct AttachArgs {
+    uint2* field_other;         // the field that is not current
+    long long n_field_words;    // its size in 8-byte words
+    uint8_t* palette;
+    int used, slots;            // palette slots in use (kept) and in total
+    const int16_t* blocks;
+    int n_blocks;
+    uint32_t* empty_bits;       // per block: 1 = id 0
+    uint32_t* pal_bits;         // per slot: 128 words of voxel occupancy
+    int* sel_claim;
+    int claim_idle;
+    int* upd_claim;
+    unsigned int* batch_counter;
+    int* blockify_state;
+    int counter;                // blockify_state[0]
+};
+hipError_t launch_world_attach(const AttachArgs& a, hipStream_t stream);
+
 size_t select_tiles(int n_probes);
 size_t select_bits_bytes(int n_probes);
 int select_claim_idle();

@@ -14,6 +14,20 @@
 //      followed by a completion event.
 // Requests are therefore overlapped across streams and packed into launches
 // as they arrive, while every request computes what it would alone.
+//
+// HBM budget.  With a budget every world lives either in the server's arena
+// (one device allocation of the budget's size, handed out first-fit with
+// coalescing) or as a pinned host snapshot.  Evictions (device -> snapshot)
+// and restores (snapshot -> arena) are copies queued on the server's copy
+// stream, in stream order with each other; a launch that needs a restored
+// world waits for the copy stream with an event, so the worker never blocks
+// on a swap and no allocation happens while serving.
+//
+// Direct mode (the plain engine): each request is its own lumx_radiance_update
+// or lumx_glossy call, with kernel arguments by value, queued on one stream in
+// arrival order as soon as its world is resident, up to kDirectDepth calls
+// deep, with no completion wait between them: the reference's one dispatch
+// per request (src/renderer/src/internal_render.cpp:1204-1206).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +35,8 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <iterator>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -49,6 +65,8 @@ struct ServedWorld {
     lumx_world* w = nullptr;
     int busy = 0;
     unsigned long long last_use = 0;
+    size_t off = 0;  // arena offset while resident (with a budget)
+    bool static_kept = false;  // its snapshot holds its static state, taken in this server
 };
 
 struct Flight {
@@ -56,6 +74,42 @@ struct Flight {
     std::vector<int> worlds;
     hipEvent_t done = nullptr;
     int stream = 0;
+};
+
+// Calls queued at once by the direct engine.
+constexpr int kDirectDepth = 256;
+
+// First-fit ranges of the arena, coalesced on release.
+struct Arena {
+    char* base = nullptr;
+    size_t bytes = 0;
+    std::map<size_t, size_t> free;  // offset -> size
+    bool take(size_t need, size_t* off) {
+        for (auto it = free.begin(); it != free.end(); ++it) {
+            if (it->second < need) continue;
+            *off = it->first;
+            const size_t rest = it->second - need, at = it->first + need;
+            free.erase(it);
+            if (rest) free[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    void give(size_t off, size_t size) {
+        auto next = free.lower_bound(off);
+        if (next != free.end() && off + size == next->first) {
+            size += next->second;
+            next = free.erase(next);
+        }
+        if (next != free.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += size;
+                return;
+            }
+        }
+        free[off] = size;
+    }
 };
 
 }  // namespace
@@ -67,6 +121,10 @@ struct lumx_server {
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> free_events;
     int next_stream = 0;
+    hipStream_t copy_stream = nullptr;  // evictions and restores
+    hipEvent_t copy_done = nullptr;     // recorded after the swaps a launch depends on
+    bool swaps_queued = false;          // swaps queued since copy_done was last recorded
+    Arena arena;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -98,14 +156,34 @@ static long long request_threads(const lumx_request& r, const lumx_world* w) {
 }
 
 // Make world i resident within the budget (caller holds the lock; `taken`
-// marks worlds already in the launch being built).  Returns false if the
-// budget cannot be met now.
+// marks worlds already in the launch being built): take an arena range,
+// evicting least-recently-used idle worlds until one fits, and queue the
+// restore on the copy stream.  Returns false if no range can be freed now.
 static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& taken) {
     ServedWorld& sw = s->worlds[i];
     if (resident(sw.w)) return true;
     const size_t need = sw.w->device_bytes;
-    const size_t budget = s->desc.hbm_budget;
-    while (budget && s->resident_bytes + need > budget) {
+    if (!s->desc.hbm_budget) {
+        // No budget: a world evicted before it was added comes back into a
+        // block of its own.
+        if (lumx_world_restore(sw.w) != LUMX_OK) return false;
+        s->resident_bytes += need;
+        s->stats.restores++;
+        return true;
+    }
+    if (!s->arena.base) {
+        // The arena: the budget, or every world when they take less.
+        size_t all = 0;
+        for (const ServedWorld& o : s->worlds) all += o.w->device_bytes;
+        const size_t bytes = std::min(s->desc.hbm_budget, all);
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return false;
+        s->arena.base = (char*)p;
+        s->arena.bytes = bytes;
+        s->arena.free[0] = bytes;
+    }
+    size_t off = 0;
+    while (!s->arena.take(need, &off)) {
         int victim = -1;
         for (int k = 0; k < (int)s->worlds.size(); ++k) {
             const ServedWorld& o = s->worlds[k];
@@ -113,18 +191,57 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
             if (victim < 0 || o.last_use < s->worlds[victim].last_use) victim = k;
         }
         if (victim < 0) return false;
-        if (lumx_world_evict(s->worlds[victim].w) != LUMX_OK) return false;
-        s->resident_bytes -= s->worlds[victim].w->device_bytes;
+        ServedWorld& v = s->worlds[victim];
+        // Serving changes only fields: after the world's first snapshot in
+        // this server only its field goes out.
+        if (world_snapshot_async(v.w, s->copy_stream, v.static_kept) != LUMX_OK) return false;
+        v.static_kept = true;
+        s->arena.give(v.off, v.w->device_bytes);
+        s->resident_bytes -= v.w->device_bytes;
         s->stats.evictions++;
+        s->swaps_queued = true;
     }
-    if (lumx_world_restore(sw.w) != LUMX_OK) return false;
+    if (world_attach_async(sw.w, s->arena.base + off, false, s->copy_stream) != LUMX_OK) {
+        s->arena.give(off, need);
+        return false;
+    }
+    sw.off = off;
     s->resident_bytes += need;
     s->stats.restores++;
+    s->swaps_queued = true;
     return true;
+}
+
+// Launches on `st` start after every swap queued so far.
+static int order_after_swaps(lumx_server* s, hipStream_t st) {
+    if (!s->swaps_queued) return LUMX_OK;
+    HIP_TRY(hipEventRecord(s->copy_done, s->copy_stream));
+    HIP_TRY(hipStreamWaitEvent(st, s->copy_done, 0));
+    s->swaps_queued = false;
+    return LUMX_OK;
+}
+
+// One request as its own single-world call (direct mode).
+static int launch_direct(lumx_server* s, Flight& f) {
+    hipStream_t st = s->streams[f.stream];
+    int rc = order_after_swaps(s, st);
+    if (rc) return rc;
+    const lumx_request& r = f.reqs[0]->spec;
+    lumx_world* w = s->worlds[f.worlds[0]].w;
+    if (r.kind == LUMX_REQ_RADIANCE)
+        rc = lumx_radiance_update(w, r.requests_dev, r.requests_dev ? r.n_requests : 0, &r.params, st);
+    else
+        rc = lumx_glossy(w, r.matnorm_dev, r.depth_dev, r.width, r.height, &r.cam, r.max_dist, r.max_steps,
+                         r.out_dev, nullptr, st);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(f.done, st));
+    return LUMX_OK;
 }
 
 static int launch_flight(lumx_server* s, Flight& f) {
     hipStream_t st = s->streams[f.stream];
+    int rc0 = order_after_swaps(s, st);
+    if (rc0) return rc0;
     std::vector<lumx_world*> rw, gw;
     std::vector<const int32_t*> rq;
     std::vector<int> rn;
@@ -171,7 +288,8 @@ static void worker_loop(lumx_server* s) {
     hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        // 1. retire finished launches
+        // 1. retire finished launches (the direct engine's are in stream
+        // order: stop at the first one still running)
         bool retired = false;
         for (size_t k = 0; k < s->flights.size();) {
             Flight& f = s->flights[k];
@@ -194,6 +312,8 @@ static void worker_loop(lumx_server* s) {
                 s->free_events.push_back(f.done);
                 s->flights.erase(s->flights.begin() + k);
                 retired = true;
+            } else if (s->desc.direct) {
+                break;
             } else {
                 ++k;
             }
@@ -203,7 +323,8 @@ static void worker_loop(lumx_server* s) {
 
         // 2. build the next launch
         bool launched = false;
-        if ((int)s->flights.size() < s->desc.streams && !s->pending.empty()) {
+        const int depth = s->desc.direct ? kDirectDepth : s->desc.streams;
+        if ((int)s->flights.size() < depth && !s->pending.empty()) {
             const int nw = (int)s->worlds.size();
             std::vector<char> taken(nw, 0), blocked(nw, 0);
             Flight f;
@@ -211,6 +332,17 @@ static void worker_loop(lumx_server* s) {
             for (auto it = s->pending.begin(); it != s->pending.end() && (int)f.reqs.size() < s->desc.max_batch;) {
                 Req* r = *it;
                 const int wi = r->spec.world;
+                if (s->desc.direct) {
+                    // Arrival order, one request per call; a world's calls
+                    // are ordered by the stream, so it need not be idle.
+                    if (!f.reqs.empty() || !ensure_resident(s, wi, taken)) break;
+                    s->worlds[wi].busy++;
+                    s->worlds[wi].last_use = ++s->use_clock;
+                    f.reqs.push_back(r);
+                    f.worlds.push_back(wi);
+                    it = s->pending.erase(it);
+                    break;
+                }
                 bool ok = !blocked[wi] && !taken[wi] && s->worlds[wi].busy == 0;
                 if (ok && !f.reqs.empty()) {
                     // one radiance format per launch
@@ -251,7 +383,7 @@ static void worker_loop(lumx_server* s) {
                     s->free_events.pop_back();
                     f.stream = s->next_stream;
                     s->next_stream = (s->next_stream + 1) % (int)s->streams.size();
-                    rc = launch_flight(s, f);
+                    rc = s->desc.direct ? launch_direct(s, f) : launch_flight(s, f);
                     s->stats.launches++;
                 }
                 if (rc != LUMX_OK) {
@@ -307,15 +439,21 @@ int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_serve
         delete s;
         return rc;
     }
-    for (int k = 0; k < d->streams; ++k) {
+    const int n_streams = d->direct ? 1 : d->streams;
+    bool ok = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&s->copy_done, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < n_streams; ++k) {
         hipStream_t st;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-            for (hipStream_t t : s->streams) hipStreamDestroy(t);
-            launcher_destroy(s->L);
-            delete s;
-            return fail(LUMX_EDEVICE, "stream creation failed");
-        }
-        s->streams.push_back(st);
+        ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+        if (ok) s->streams.push_back(st);
+    }
+    if (!ok) {
+        for (hipStream_t t : s->streams) hipStreamDestroy(t);
+        if (s->copy_stream) hipStreamDestroy(s->copy_stream);
+        if (s->copy_done) hipEventDestroy(s->copy_done);
+        launcher_destroy(s->L);
+        delete s;
+        return fail(LUMX_EDEVICE, "stream creation failed");
     }
     s->worker = std::thread(worker_loop, s);
     *out = s;
@@ -335,9 +473,31 @@ int lumx_server_destroy(lumx_server_t s) {
         hipStreamSynchronize(t);
         hipStreamDestroy(t);
     }
+    hipStreamSynchronize(s->copy_stream);
+    int status = LUMX_OK;
+    if (s->arena.base) {
+        // Worlds in the arena move into blocks of their own (the same layout,
+        // so one device copy each); a world that cannot stays on the host.
+        for (ServedWorld& sw : s->worlds) {
+            if (!resident(sw.w) || sw.w->dev_own) continue;
+            void* mem = nullptr;
+            if (hipMalloc(&mem, sw.w->device_bytes) == hipSuccess &&
+                hipMemcpy(mem, sw.w->dev_mem, sw.w->device_bytes, hipMemcpyDeviceToDevice) == hipSuccess) {
+                world_rebase(sw.w, mem, true);
+            } else {
+                if (mem) hipFree(mem);
+                if (world_snapshot_async(sw.w, s->copy_stream, sw.static_kept) != LUMX_OK) status = LUMX_EDEVICE;
+                hipStreamSynchronize(s->copy_stream);
+            }
+        }
+        hipFree(s->arena.base);
+    }
+    hipStreamDestroy(s->copy_stream);
+    hipEventDestroy(s->copy_done);
     for (hipEvent_t e : s->free_events) hipEventDestroy(e);
     launcher_destroy(s->L);
     delete s;
+    if (status) return fail(status, "a world could not leave the server's arena");
     return LUMX_OK;
 }
 
@@ -347,19 +507,25 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
     std::lock_guard<std::mutex> lk(s->mu);
     for (const ServedWorld& o : s->worlds)
         if (o.w == w) return fail(LUMX_EINVAL, "world already added");
-    ServedWorld sw;
-    sw.w = w;
-    s->worlds.push_back(sw);
-    if (resident(w)) {
-        s->resident_bytes += w->device_bytes;
-        if (s->desc.hbm_budget && s->resident_bytes > s->desc.hbm_budget) {
-            // Over budget: this world waits on the host until a request needs it.
-            if (lumx_world_evict(w) == LUMX_OK) {
-                s->resident_bytes -= w->device_bytes;
-                s->stats.evictions++;
-            }
+    if (s->desc.hbm_budget) {
+        if (w->device_bytes > s->desc.hbm_budget)
+            return fail(LUMX_EINVAL, "world needs %zu bytes, more than the budget of %zu", w->device_bytes,
+                        s->desc.hbm_budget);
+        if (s->arena.base) return fail(LUMX_EINVAL, "worlds must be added before the first request");
+        // With a budget every world is served from the arena: it starts on
+        // the host, in its pinned snapshot.
+        if (resident(w)) {
+            HIP_TRY(hipSetDevice(s->ctx->device));
+            int rc = lumx_world_evict(w);
+            if (rc) return rc;
         }
     }
+    ServedWorld sw;
+    sw.w = w;
+    // With a budget the world is on the host now, in a full snapshot.
+    sw.static_kept = s->desc.hbm_budget != 0 && w->snap_static_valid;
+    s->worlds.push_back(sw);
+    if (resident(w)) s->resident_bytes += w->device_bytes;
     *id = (int)s->worlds.size() - 1;
     return LUMX_OK;
 }

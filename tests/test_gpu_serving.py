@@ -72,7 +72,7 @@ def run_alone(ctx, plan):
     return fields, outs
 
 
-def run_served(ctx, plan, budget=0, max_batch=8, streams=2, block=0, rays_per_lane=0):
+def run_served(ctx, plan, budget=0, max_batch=8, streams=2, block=0, rays_per_lane=0, direct=False):
     import torch
     worlds, mats = [], []
     for i, s in enumerate(SIZES):
@@ -81,7 +81,7 @@ def run_served(ctx, plan, budget=0, max_batch=8, streams=2, block=0, rays_per_la
         mats.append(m)
     torch.cuda.synchronize()
     srv = Server(ctx, worlds, hbm_budget=budget, max_batch=max_batch, streams=streams, block=block,
-                 rays_per_lane=rays_per_lane)
+                 rays_per_lane=rays_per_lane, direct=direct)
     outs, tickets = [], []
     for kind, wi, a in plan:
         if kind == "r":
@@ -156,6 +156,38 @@ def test_plain_mode_one_request_per_launch(gpu_ctx):
     assert stats["launches"] == len(plan)
     for a, b in zip(fields, ref_fields):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("budget_worlds", [0, 2.2])
+def test_direct_engine_equals_alone(gpu_ctx, budget_worlds):
+    """The plain engine (one single-world call per request, queued with no
+    wait between calls), with and without a budget that forces swaps."""
+    plan = request_plan(8, 30, _mats())
+    ref_fields, ref_outs = run_alone(gpu_ctx, plan)
+    w, _ = make(gpu_ctx, SIZES[3], 1)
+    per_world = w.resident()[1]
+    w.close()
+    budget = int(budget_worlds * per_world)
+    fields, outs, stats, lat = run_served(gpu_ctx, plan, budget=budget, direct=True)
+    assert stats["launches"] == len(plan) and stats["completed"] == len(plan)
+    if budget:
+        assert stats["evictions"] > 0 and stats["restores"] > 0
+        assert stats["resident_bytes"] <= budget
+    for a, b in zip(fields, ref_fields):
+        assert np.array_equal(a, b)
+    for a, b in zip(outs, ref_outs):
+        if a is not None:
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    assert min(lat) > 0
+
+
+def test_budget_rejects_a_world_larger_than_itself(gpu_ctx):
+    w, _ = make(gpu_ctx, (16, 16, 8), 2)
+    try:
+        with pytest.raises(Exception, match="budget"):
+            Server(gpu_ctx, [w], hbm_budget=w.resident()[1] // 2)
+    finally:
+        w.close()
 
 
 def test_evict_restore_roundtrip(gpu_ctx):
