@@ -21,9 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction
-# per SIMD every 4 cycles (16 lanes wide), at the 2.4 GHz peak engine clock.
-VALU_PEAK_GINST = 1024 * 2.4e9 / 4 / 1e9
+# VALU issue peak of MI355X in wave64 instructions/s: the published FP32
+# vector peak (157.3 TFLOP/s) over 2 flops per FMA lane and 64 lanes, i.e. one
+# wave64 VALU instruction per SIMD every 2 cycles on 1,024 SIMDs at 2.4 GHz.
+VALU_PEAK_GINST = 157.3e12 / 2 / 64 / 1e9
 
 
 def measured_traffic(kernel):
@@ -763,6 +764,16 @@ def run_chain(args):
         "map_voxels_per_s": map_threads / (per["map"] / 1e3),
         "glossy_rays_per_s": glossy_px / (per["glossy"] / 1e3),
     }
+    # map.comp's byte model: per thread an int16 block id and a u8 model voxel,
+    # per non-zero model voxel one u8 palette store; against the device map
+    # pass (blockify, copies, bits and map kernels, HIP events of the library).
+    map_bytes = 3 * map_threads + int(sum(int((v != 0).sum()) for v in vox))
+    dev_map_ms = res["device_map_chain"]["pass_ms"]["map"]
+    res["map"] = {"threads": map_threads, "bytes_model": map_bytes, "device_pass_ms": dev_map_ms,
+                  "GBps_at_pass_time": map_bytes / (dev_map_ms / 1e3) / 1e9,
+                  "frac_of_hbm_peak_at_pass_time": map_bytes / (dev_map_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    if args.copy_calibration_mb > 0:
+        res["copy_calibration_bytes"] = copy_calibration(args.copy_calibration_mb)
     for m in ms:
         m.close()
     w.close()
@@ -857,8 +868,8 @@ def make_parser():
     ap.add_argument("--serving-worlds", type=int, default=200)
     ap.add_argument("--requests", type=int, default=0,
                     help="serving: timed requests (default 20 per step, 4000 at the default --steps)")
-    ap.add_argument("--max-batch", type=int, default=64)
-    ap.add_argument("--streams", type=int, default=2)
+    ap.add_argument("--max-batch", type=int, default=16, help="serving: requests per launch")
+    ap.add_argument("--streams", type=int, default=4, help="serving: launches in flight")
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
                     help="serving: HBM budget in GiB (default: --hbm-budget-frac of the worlds' device state)")
     ap.add_argument("--hbm-budget-frac", type=float, default=0.5,
