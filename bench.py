@@ -783,6 +783,87 @@ def run_chain(args):
     return res
 
 
+def run_frames(args):
+    """The drop-in frame loop: the chain world (48x48x16, 64-block palette,
+    32 models of 16^3-64^3) and a 1920x1080 G-buffer driven through the C99
+    API (include/clum.h) as a C host drives it: per frame start, one draw per
+    model, prepare, submit.  Frames run map -> shift -> radiance (device probe
+    selection, counted dispatch) -> glossy with two frames in flight; the
+    timed region is K submissions followed by lum_wait_idle."""
+    import ctypes as C
+    import torch
+    from lum_amd import synth, clum
+
+    cfg = synth.PLAIN
+    rng = np.random.default_rng(cfg["seed"])
+    size = cfg["size"]
+    static = cfg["n_palette"]
+    blocks = synth.make_world(size, rng, static)
+    pal = synth.make_palette(rng, static)
+    mats = synth.make_materials(rng)
+    synth.make_field(size, rng)
+    vox, quats, shifts = synth.make_models(rng, n=32, lo=16, hi=64, fill=0.3, world_size=size)
+    W, H = 1920, 1080
+    mn, dp = synth.make_gbuffer(W, H, mats, rng)
+    mn_d, dp_d = torch.from_numpy(mn).cuda(), torch.from_numpy(dp).cuda()
+    out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    L = clum.lib()
+    s = clum.settings(size, static)
+    lum = L.lum_create_instance(C.byref(s))
+
+    def ptr(a):
+        return a.ctypes.data_as(C.c_void_p)
+
+    L.lum_init(lum, C.byref(s))
+    b16 = np.ascontiguousarray(blocks, np.int16)
+    L.lum_load_world(lum, ptr(b16))
+    for b in range(1, static):
+        L.lum_load_block(lum, b, ptr(np.ascontiguousarray(pal[b].transpose(2, 1, 0))))
+    L.lum_upload_block_palette_to_gpu(lum)
+    m32 = np.ascontiguousarray(mats, np.float32)
+    L.lumx_lum_set_materials(lum, ptr(m32))
+    L.lum_upload_material_palette_to_gpu(lum)
+    keep = [np.ascontiguousarray(v) for v in vox]
+    meshes = [L.lum_load_mesh_data(lum, ptr(v), v.shape[2], v.shape[1], v.shape[0]) for v in keep]
+    trans = [clum.transform(q, sh) for q, sh in zip(quats, shifts)]
+    L.lumx_lum_set_gbuffer(lum, C.c_void_p(mn_d.data_ptr()), C.c_void_p(dp_d.data_ptr()), W, H,
+                           C.c_void_p(out.data_ptr()))
+
+    def frame():
+        L.lum_start_frame(lum)
+        for m, t in zip(meshes, trans):
+            L.lum_draw_model(lum, m, C.byref(t))
+        L.lum_prepare_frame(lum)
+        L.lum_submit_frame(lum)
+
+    for _ in range(max(args.warmup, 2)):
+        frame()
+    L.lum_wait_idle(lum)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame()
+    t_submit = time.perf_counter() - t0
+    L.lum_wait_idle(lum)
+    wall = time.perf_counter() - t0
+    counter = C.c_int()
+    L.lumx_world_palette_counter(L.lumx_lum_world(lum), C.byref(counter))
+    passes = {"map_ms": L.lum_get_timestamp_difference(lum, 0, 1),
+              "radiance_ms": L.lum_get_timestamp_difference(lum, 1, 2),
+              "glossy_ms": L.lum_get_timestamp_difference(lum, 2, 3)}
+    L.lum_cleanup(lum)
+    return {
+        "metric": "frames/s of the clum.h drop-in frame (map -> radiance -> glossy), 48x48x16 world, 32 models, "
+                  "1920x1080 glossy",
+        "value": args.steps / wall, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded world, palette, materials, 32 models, G-buffer)",
+        "config": {"workload": "frames: lum_start_frame / 32 x lum_draw_model / lum_prepare_frame / "
+                               "lum_submit_frame, two frames in flight", "palette_slots_used": counter.value},
+        "host_submit_ms_per_frame": t_submit * 1e3 / args.steps,
+        "pass_ms_of_a_finished_frame": passes,
+    }
+
+
 def run_precision(args):
     """The precision workload: a 256x256x32-block world (2,097,152 probes),
     128 rays per probe, max 96 steps, 8 accumulated frames, radiance stored in
@@ -865,7 +946,8 @@ def make_parser():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="serving", choices=["serving", "plain", "batched", "chain", "precision"])
+    ap.add_argument("--workload", default="serving",
+                    choices=["serving", "plain", "batched", "chain", "precision", "frames"])
     ap.add_argument("--frames", type=int, default=8)
     ap.add_argument("--serving-worlds", type=int, default=200)
     ap.add_argument("--requests", type=int, default=0,
@@ -902,6 +984,8 @@ def main():
         out = run_chain(args)
     elif args.workload == "precision":
         out = run_precision(args)
+    elif args.workload == "frames":
+        out = run_frames(args)
     elif args.workload == "plain":
         out = run_plain(args)
     else:

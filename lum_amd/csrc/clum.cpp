@@ -48,9 +48,17 @@ struct Renderer {
     float radiance_ms = 0.0f;
     int32_t* req_dev = nullptr;      // probe list of update_radiance (n_blocks entries)
     int32_t* count_dev = nullptr;    // its length, written by the device selection
-    int32_t* special_dev = nullptr;  // specials of the frame being submitted
+    int32_t* special_dev = nullptr;  // specials of the frame being submitted (stream-ordered)
     size_t special_cap = 0;          // entries special_dev holds
-    std::vector<int32_t> special_up; // host source of special_dev until the frame ends
+    // Frames in flight (MAX_FRAMES_IN_FLIGHT = 2, internal_types.hpp:20): per
+    // slot the frame's timing events and the pinned staging of its specials,
+    // reused once the frame two submissions back has finished.
+    struct Slot {
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // start, map, radiance, glossy (= done)
+        bool pending = false;
+        int32_t* special_pin = nullptr;
+        size_t special_pin_cap = 0;
+    } slot[2];
     int radiance_shift[3] = {0, 0, 0};  // Renderer::stored_radiance_shift
     bool has_palette = false;           // a Material table was taken from a .vox file
     const uint8_t* gb_matnorm = nullptr;
@@ -264,6 +272,16 @@ static void release(Renderer* r) {
     if (r->special_dev) hipFree(r->special_dev);
     r->special_dev = nullptr;
     r->special_cap = 0;
+    for (Renderer::Slot& fs : r->slot) {
+        for (hipEvent_t& e : fs.ev) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+        if (fs.special_pin) hipHostFree(fs.special_pin);
+        fs.special_pin = nullptr;
+        fs.special_pin_cap = 0;
+        fs.pending = false;
+    }
     if (r->ctx) lumx_context_destroy(r->ctx);
     r->ctx = nullptr;
 }
@@ -300,6 +318,9 @@ void lum_init(LumRenderer instance, const LumSettings* settings) {
     // Every probe appears at most once in the selection: n_blocks entries suffice.
     if (hipMalloc(&r->req_dev, nb * 16) != hipSuccess) r->req_dev = nullptr;
     if (hipMalloc(&r->count_dev, sizeof(int32_t)) != hipSuccess) r->count_dev = nullptr;
+    for (Renderer::Slot& fs : r->slot)
+        for (hipEvent_t& e : fs.ev)
+            if (hipEventCreate(&e) != hipSuccess) e = nullptr;
 }
 
 void lum_load_world(LumRenderer instance, const LumMeshBlock* world_data) {
@@ -435,33 +456,62 @@ void lum_prepare_frame(LumRenderer instance) {
                      [](const DrawRequest& a, const DrawRequest& b) { return a.cam_dist > b.cam_dist; });
 }
 
+// One frame: map.comp with blockify on the device, shift_radiance,
+// update_radiance (device probe selection, work-size controller, dispatch
+// over the device-side count) and glossy, queued on the context's stream with
+// no wait for the GPU.  Two frames are in flight (renderer.cpp:71-73): a
+// submission first waits for the frame two submissions back, the reference's
+// fence wait, and takes its pass times for the timestamps and the controller.
 void lum_submit_frame(LumRenderer instance) {
     Renderer* r = get(instance);
     if (!r || !r->world || !r->req_dev || !r->count_dev) return;
+    Renderer::Slot& fs = r->slot[r->frame & 1];
+    for (hipEvent_t e : fs.ev)
+        if (!e) return;
+    if (fs.pending) {
+        if (hipEventSynchronize(fs.ev[3]) != hipSuccess) return;
+        float t[3] = {0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < 3; ++k) (void)hipEventElapsedTime(&t[k], fs.ev[k], fs.ev[k + 1]);
+        r->radiance_ms = t[1];
+        r->ts[0] = 0.0;
+        r->ts[1] = t[0];
+        r->ts[2] = r->ts[1] + t[1];
+        r->ts[3] = r->ts[2] + t[2];
+        fs.pending = false;
+    }
     update_camera(r->cam);
-    void* stream = lumx_context_stream(r->ctx);
-    // blockify + exec_copies + map.comp
+    hipStream_t stream = (hipStream_t)lumx_context_stream(r->ctx);
+    if (hipEventRecord(fs.ev[0], stream) != hipSuccess) return;
+    // blockify + exec_copies + map.comp, blockify on the device: only the
+    // model boxes cross PCIe (the origin world too, after it changed).
     std::vector<lumx_model_t> models;
-    std::vector<lumx_transform> trans;
+    std::vector<lumx_placement> placed;
     for (const DrawRequest& d : r->mesh_que) {
-        models.push_back(d.model);
         lumx_transform t;
         std::memcpy(t.rot, d.trans.rot, sizeof(t.rot));
         std::memcpy(t.shift, d.trans.shift, sizeof(t.shift));
-        trans.push_back(t);
+        int size[3];
+        lumx_placement p;
+        if (lumx_model_size(d.model, size) != LUMX_OK || lumx_model_placement(&t, size, &p) != LUMX_OK) return;
+        models.push_back(d.model);
+        placed.push_back(p);
     }
-    if (lumx_map_models(r->world, (int)models.size(), models.data(), trans.data(), stream) != LUMX_OK) return;
-    float map_ms = 0.0f;
-    lumx_world_last_pass_ms(r->world, 1, &map_ms);
+    if (lumx_map_models_device(r->world, (int)models.size(), models.data(), placed.data(), stream) != LUMX_OK) return;
+    if (hipEventRecord(fs.ev[1], stream) != hipSuccess) return;
     // shift_radiance with the stored shift (renderer.cpp:436), then
-    // update_radiance: probe selection on the device, work-size controller,
-    // dispatch over the device-side count (no host round trip).
+    // update_radiance.
     if (lumx_world_shift_radiance(r->world, r->radiance_shift[0], r->radiance_shift[1], r->radiance_shift[2],
                                   stream) != LUMX_OK)
         return;
-    r->special_up.swap(r->special);
-    r->special.clear();
-    const size_t n_special = r->special_up.size() / 4;
+    const size_t n_special = r->special.size() / 4;
+    if (n_special > fs.special_pin_cap) {
+        // The slot's previous frame has finished (waited for above).
+        if (fs.special_pin) hipHostFree(fs.special_pin);
+        fs.special_pin = nullptr;
+        fs.special_pin_cap = 0;
+        if (hipHostMalloc((void**)&fs.special_pin, n_special * 16, hipHostMallocDefault) != hipSuccess) return;
+        fs.special_pin_cap = n_special;
+    }
     if (n_special > r->special_cap) {
         if (r->special_dev) hipFree(r->special_dev);
         r->special_cap = 0;
@@ -471,13 +521,18 @@ void lum_submit_frame(LumRenderer instance) {
         }
         r->special_cap = n_special;
     }
-    // special_up stays untouched until the stream synchronisation below.
-    if (n_special && hipMemcpyAsync(r->special_dev, r->special_up.data(), n_special * 16, hipMemcpyHostToDevice,
-                                    (hipStream_t)stream) != hipSuccess)
-        return;
+    if (n_special) {
+        std::memcpy(fs.special_pin, r->special.data(), n_special * 16);
+        if (hipMemcpyAsync(r->special_dev, fs.special_pin, n_special * 16, hipMemcpyHostToDevice, stream) !=
+            hipSuccess)
+            return;
+    }
+    r->special.clear();
     if (lumx_select_probes_device(r->world, r->special_dev, (int)n_special, r->req_dev, r->count_dev, stream) !=
         LUMX_OK)
         return;
+    // Work-size controller on the radiance time of the last finished frame
+    // (internal_render.cpp:1192-1196).
     if (r->radiance_ms < 1.8f) r->magic_size--;
     else if (r->radiance_ms > 2.2f) r->magic_size++;
     r->magic_size = std::min(std::max(r->magic_size, 1), 10);
@@ -493,22 +548,16 @@ void lum_submit_frame(LumRenderer instance) {
     lumx_world_info(r->world, &wd);
     const int n_blocks = wd.size[0] * wd.size[1] * wd.size[2];
     if (lumx_radiance_update_counted(r->world, r->req_dev, r->count_dev, n_blocks, &p, stream) != LUMX_OK) return;
-    bool glossy_ok = false;
+    if (hipEventRecord(fs.ev[2], stream) != hipSuccess) return;
     if (r->gb_matnorm && r->gb_depth && r->gb_out) {
         lumx_camera cam;
         lumx_camera_default(r->gb_w, r->gb_h, &cam);
-        glossy_ok = lumx_glossy(r->world, r->gb_matnorm, r->gb_depth, r->gb_w, r->gb_h, &cam, 128.0f, 0, r->gb_out,
-                                nullptr, stream) == LUMX_OK;
+        if (lumx_glossy(r->world, r->gb_matnorm, r->gb_depth, r->gb_w, r->gb_h, &cam, 128.0f, 0, r->gb_out, nullptr,
+                        stream) != LUMX_OK)
+            return;
     }
-    // Pass times once the frame's work is queued (each read waits for its pass).
-    lumx_world_last_pass_ms(r->world, 0, &r->radiance_ms);
-    float glossy_ms = 0.0f;
-    if (glossy_ok) lumx_world_last_pass_ms(r->world, 2, &glossy_ms);
-    (void)hipStreamSynchronize((hipStream_t)stream);
-    r->ts[0] = 0.0;
-    r->ts[1] = map_ms;
-    r->ts[2] = r->ts[1] + r->radiance_ms;
-    r->ts[3] = r->ts[2] + glossy_ms;
+    if (hipEventRecord(fs.ev[3], stream) != hipSuccess) return;
+    fs.pending = true;
     r->frame++;
 }
 

@@ -16,8 +16,9 @@
 // as they arrive, while every request computes what it would alone.
 //
 // HBM budget.  With a budget every world lives either in the server's arena
-// (one device allocation of the budget's size, handed out first-fit with
-// coalescing) or as a pinned host snapshot.  Evictions (device -> snapshot)
+// (one device allocation of the budget's size, made at creation and handed
+// out first-fit with coalescing) or as a pinned host snapshot; worlds added
+// while the arena has room move into it.  Evictions (device -> snapshot)
 // and restores (snapshot -> arena) are copies queued on the server's copy
 // stream, in stream order with each other; a launch that needs a restored
 // world waits for the copy stream with an event, so the worker never blocks
@@ -95,6 +96,18 @@ struct Arena {
         }
         return false;
     }
+    // Takes [off, off + need) if it lies inside one free range.
+    bool take_at(size_t off, size_t need) {
+        auto it = free.upper_bound(off);
+        if (it == free.begin()) return false;
+        --it;
+        const size_t f0 = it->first, f1 = it->first + it->second;
+        if (off < f0 || off + need > f1) return false;
+        free.erase(it);
+        if (off > f0) free[f0] = off - f0;
+        if (f1 > off + need) free[off + need] = f1 - (off + need);
+        return true;
+    }
     void give(size_t off, size_t size) {
         auto next = free.lower_bound(off);
         if (next != free.end() && off + size == next->first) {
@@ -156,9 +169,12 @@ static long long request_threads(const lumx_request& r, const lumx_world* w) {
 }
 
 // Make world i resident within the budget (caller holds the lock; `taken`
-// marks worlds already in the launch being built): take an arena range,
-// evicting least-recently-used idle worlds until one fits, and queue the
-// restore on the copy stream.  Returns false if no range can be freed now.
+// marks worlds already in the launch being built).  The world needs one
+// contiguous arena range: among the windows of that size made only of free
+// space and idle resident worlds, the one whose eviction copies the fewest
+// bytes (then: whose worlds were used least recently) is cleared and taken.
+// Nothing is evicted when no such window exists (busy worlds split the
+// arena); the request then waits.  The restore is queued on the copy stream.
 static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& taken) {
     ServedWorld& sw = s->worlds[i];
     if (resident(sw.w)) return true;
@@ -171,35 +187,66 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
         s->stats.restores++;
         return true;
     }
-    if (!s->arena.base) {
-        // The arena: the budget, or every world when they take less.
-        size_t all = 0;
-        for (const ServedWorld& o : s->worlds) all += o.w->device_bytes;
-        const size_t bytes = std::min(s->desc.hbm_budget, all);
-        void* p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) return false;
-        s->arena.base = (char*)p;
-        s->arena.bytes = bytes;
-        s->arena.free[0] = bytes;
-    }
     size_t off = 0;
-    while (!s->arena.take(need, &off)) {
-        int victim = -1;
+    if (!s->arena.take(need, &off)) {
+        // The arena as segments in address order: free gaps (k = -1) and
+        // resident worlds; pinned ones (busy, taken) end a window.
+        struct Seg {
+            size_t off, size;
+            int k;
+        };
+        std::vector<Seg> seg;
+        for (const auto& f : s->arena.free) seg.push_back({f.first, f.second, -1});
         for (int k = 0; k < (int)s->worlds.size(); ++k) {
             const ServedWorld& o = s->worlds[k];
-            if (k == i || o.busy || taken[k] || !resident(o.w)) continue;
-            if (victim < 0 || o.last_use < s->worlds[victim].last_use) victim = k;
+            if (resident(o.w) && !o.w->dev_own) seg.push_back({o.off, o.w->device_bytes, k});
         }
-        if (victim < 0) return false;
-        ServedWorld& v = s->worlds[victim];
-        // Serving changes only fields: after the world's first snapshot in
-        // this server only its field goes out.
-        if (world_snapshot_async(v.w, s->copy_stream, v.static_kept) != LUMX_OK) return false;
-        v.static_kept = true;
-        s->arena.give(v.off, v.w->device_bytes);
-        s->resident_bytes -= v.w->device_bytes;
-        s->stats.evictions++;
-        s->swaps_queued = true;
+        std::sort(seg.begin(), seg.end(), [](const Seg& x, const Seg& y) { return x.off < y.off; });
+        auto pinned = [&](const Seg& g) {
+            if (g.k < 0) return false;
+            const ServedWorld& o = s->worlds[g.k];
+            return g.k == i || o.busy > 0 || taken[g.k];
+        };
+        int best = -1;
+        size_t best_cost = 0;
+        unsigned long long best_age = 0;
+        for (int a = 0; a < (int)seg.size(); ++a) {
+            if (pinned(seg[a])) continue;
+            size_t got = 0, cost = 0;
+            unsigned long long age = 0;
+            int b = a;
+            for (; b < (int)seg.size() && got < need; ++b) {
+                // The segments tile the arena; a pinned world ends the window.
+                if (pinned(seg[b]) || (b > a && seg[b].off != seg[b - 1].off + seg[b - 1].size)) break;
+                got = seg[b].off + seg[b].size - seg[a].off;
+                if (seg[b].k >= 0) {
+                    cost += seg[b].size;
+                    age = std::max(age, s->worlds[seg[b].k].last_use);
+                }
+            }
+            if (got < need) continue;
+            if (best < 0 || cost < best_cost || (cost == best_cost && age < best_age)) {
+                best = a;
+                best_cost = cost;
+                best_age = age;
+            }
+        }
+        if (best < 0) return false;
+        const size_t start = seg[best].off;
+        for (int b = best; b < (int)seg.size() && seg[b].off < start + need; ++b) {
+            if (seg[b].k < 0) continue;
+            ServedWorld& v = s->worlds[seg[b].k];
+            // Serving changes only fields: after the world's first snapshot in
+            // this server only its field goes out.
+            if (world_snapshot_async(v.w, s->copy_stream, v.static_kept) != LUMX_OK) return false;
+            v.static_kept = true;
+            s->arena.give(v.off, v.w->device_bytes);
+            s->resident_bytes -= v.w->device_bytes;
+            s->stats.evictions++;
+            s->swaps_queued = true;
+        }
+        if (!s->arena.take_at(start, need)) return false;
+        off = start;
     }
     if (world_attach_async(sw.w, s->arena.base + off, false, s->copy_stream) != LUMX_OK) {
         s->arena.give(off, need);
@@ -442,6 +489,20 @@ int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_serve
     const int n_streams = d->direct ? 1 : d->streams;
     bool ok = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&s->copy_done, hipEventDisableTiming) == hipSuccess;
+    if (ok && d->hbm_budget) {
+        // The arena: every byte of world state the server may hold.
+        void* p = nullptr;
+        if (hipMalloc(&p, d->hbm_budget) != hipSuccess) {
+            if (s->copy_stream) hipStreamDestroy(s->copy_stream);
+            if (s->copy_done) hipEventDestroy(s->copy_done);
+            launcher_destroy(s->L);
+            delete s;
+            return fail(LUMX_ENOMEM, "HBM budget of %zu bytes could not be allocated", d->hbm_budget);
+        }
+        s->arena.base = (char*)p;
+        s->arena.bytes = d->hbm_budget;
+        s->arena.free[0] = d->hbm_budget;
+    }
     for (int k = 0; ok && k < n_streams; ++k) {
         hipStream_t st;
         ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
@@ -451,6 +512,7 @@ int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_serve
         for (hipStream_t t : s->streams) hipStreamDestroy(t);
         if (s->copy_stream) hipStreamDestroy(s->copy_stream);
         if (s->copy_done) hipEventDestroy(s->copy_done);
+        if (s->arena.base) hipFree(s->arena.base);
         launcher_destroy(s->L);
         delete s;
         return fail(LUMX_EDEVICE, "stream creation failed");
@@ -511,19 +573,35 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
         if (w->device_bytes > s->desc.hbm_budget)
             return fail(LUMX_EINVAL, "world needs %zu bytes, more than the budget of %zu", w->device_bytes,
                         s->desc.hbm_budget);
-        if (s->arena.base) return fail(LUMX_EINVAL, "worlds must be added before the first request");
-        // With a budget every world is served from the arena: it starts on
-        // the host, in its pinned snapshot.
-        if (resident(w)) {
-            HIP_TRY(hipSetDevice(s->ctx->device));
-            int rc = lumx_world_evict(w);
-            if (rc) return rc;
-        }
     }
     ServedWorld sw;
     sw.w = w;
-    // With a budget the world is on the host now, in a full snapshot.
-    sw.static_kept = s->desc.hbm_budget != 0 && w->snap_static_valid;
+    if (s->desc.hbm_budget && resident(w)) {
+        // With a budget every world is served from the arena: a resident
+        // world moves there while the arena has room (one device copy of its
+        // block), else to the host, into its pinned snapshot.
+        HIP_TRY(hipSetDevice(s->ctx->device));
+        HIP_TRY(hipDeviceSynchronize());  // work on the world may be queued on any stream
+        // Its pinned snapshot now, so no eviction allocates while serving.
+        int rc = world_snapshot_reserve(w, std::max(w->counter, w->prev_counter), s->copy_stream);
+        if (rc) return rc;
+        size_t off = 0;
+        if (s->arena.take(w->device_bytes, &off)) {
+            hipError_t e = hipMemcpy(s->arena.base + off, w->dev_mem, w->device_bytes, hipMemcpyDeviceToDevice);
+            if (e != hipSuccess) {
+                s->arena.give(off, w->device_bytes);
+                return fail(LUMX_EDEVICE, "moving a world into the arena: %s", hipGetErrorString(e));
+            }
+            if (w->dev_own) hipFree(w->dev_mem);
+            world_rebase(w, s->arena.base + off, false);
+            sw.off = off;
+        } else {
+            rc = lumx_world_evict(w);
+            if (rc) return rc;
+        }
+    }
+    // A world on the host holds a full snapshot.
+    sw.static_kept = s->desc.hbm_budget != 0 && !resident(w) && w->snap_static_valid;
     s->worlds.push_back(sw);
     if (resident(w)) s->resident_bytes += w->device_bytes;
     *id = (int)s->worlds.size() - 1;
