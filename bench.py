@@ -146,6 +146,7 @@ def run_plain(args):
         sweep = []
         for variant, threads, P, eb, pb, k in ([(0, 0, 0, 0, 0, 1), (0, 0, 0, 0, 0, 2), (0, 0, 0, 0, 0, 4),
                                                  (2, 0, 0, 0, 0, 1)] +
+                                                [(3, 0, P, 0, 0, 1) for P in (1, 2, 4) if rays == 64] +
                                                 [(1, th, P, eb, pb, 1) for th in (256, 512, 1024) for P in (4, 16, 64)
                                                  for eb in (0, 1) for pb in (0, 1)]):
             if P * rays > 4096:

@@ -129,9 +129,12 @@ typedef struct {
  * replaces the fixed 64-lane workgroup of radiance.comp:37,360. */
 typedef struct {
     int variant;            /* 0 = one thread per ray, a probe per wave; 1 = tiled persistent;
-                               2 = as 0, marching the occupancy bits instead of the u8 palette */
+                               2 = as 0, marching the occupancy bits instead of the u8 palette;
+                               3 = lane refill: a wave marches the rays of probes_per_batch (1..4)
+                               probes as one pool, finished lanes taking the next ray (64 rays) */
     int threads;            /* workgroup size of the tiled kernel (multiple of 64, <= 1024) */
-    int probes_per_batch;   /* probes per batch of the tiled kernel (power of two, 1..64) */
+    int probes_per_batch;   /* probes per batch of the tiled kernel (power of two, 1..64); probes per
+                               wave of the refill variant */
     int lds_block_bits;     /* stage the empty-block bits in LDS when they fit */
     int lds_palette_bits;   /* stage the palette occupancy bits in LDS when they fit */
     int wg_per_cu;          /* persistent workgroups per CU, 0 = occupancy limit */
@@ -350,11 +353,10 @@ int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
  * flight while it admits the next.  Each request computes exactly what its
  * own lumx_radiance_update / lumx_glossy call would.  Worlds added to a
  * server belong to it until it is destroyed.
- * With an HBM budget the server holds one device arena of the budget's size
- * (or of all its worlds' bytes, if less) and every world lives there or in a
- * pinned host snapshot: lumx_server_add_world moves a resident world to the
- * host, a world larger than the budget is LUMX_EINVAL, and worlds are added
- * before the first request.  Evictions and restores are copies on the
+ * With an HBM budget the server allocates one device arena of the budget's
+ * size at creation and every world lives there or in a pinned host snapshot:
+ * lumx_server_add_world moves a resident world into the arena while it has
+ * room, else to the host; a world larger than the budget is LUMX_EINVAL.  Evictions and restores are copies on the
  * server's own stream, never a host wait.  At destroy, worlds in the arena
  * move into device blocks of their own; evicted ones stay on the host
  * (lumx_world_restore brings them back). */

@@ -1329,9 +1329,12 @@ int lumx::radiance_params_check(const lumx_radiance_params* p) {
 }  // extern "C++"
 
 static int check_tuning(const lumx_world* w, const lumx_radiance_tuning& t, int rays) {
-    if (t.variant < 0 || t.variant > 2) return fail(LUMX_EINVAL, "tuning variant %d invalid", t.variant);
+    if (t.variant < 0 || t.variant > 3) return fail(LUMX_EINVAL, "tuning variant %d invalid", t.variant);
     if (t.rays_per_lane != 0 && t.rays_per_lane != 1 && t.rays_per_lane != 2 && t.rays_per_lane != 4)
         return fail(LUMX_EINVAL, "tuning rays_per_lane %d is not 1, 2 or 4", t.rays_per_lane);
+    if (t.variant == 3 && (t.probes_per_batch < 1 || t.probes_per_batch > 4))
+        return fail(LUMX_EINVAL, "tuning probes_per_batch %d invalid for the refill variant (1..4)",
+                    t.probes_per_batch);
     if (t.variant != 1) return LUMX_OK;
     if (t.threads < 64 || t.threads > 1024 || t.threads % 64)
         return fail(LUMX_EINVAL, "tuning threads %d invalid", t.threads);
@@ -1397,7 +1400,14 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
     a.ray_steps = steps;
     // The scatter that applies a subset update consumes (and resets) the claims.
     if (a.staged && apply) a.claim = w->upd_claim;
-    if (tune.variant == 0 || tune.variant == 2) {
+    if (tune.variant == 3 && !traced) {
+        if (p->rays != 64) return fail(LUMX_EINVAL, "the refill variant takes 64 rays per probe");
+        HIP_TRY(launch_radiance_refill(a, w->desc.radiance_format, tune.probes_per_batch, s));
+        if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
+        return LUMX_OK;
+    }
+    if (tune.variant == 0 || tune.variant == 2 || tune.variant == 3) {
+        // (a traced launch of the refill variant traces with the lane kernel: the same results)
         HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, tune.variant == 2, tune.rays_per_lane, s));
         if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
         return LUMX_OK;
@@ -1477,6 +1487,13 @@ int lumx_autotune_radiance(lumx_world_t w, const lumx_radiance_params* p, int re
     c0.rays_per_lane = 0;
     c0.variant = 2;
     cands.push_back(c0);
+    if (p->rays == 64)
+        for (int P : {1, 2, 4}) {
+            lumx_radiance_tuning c{};
+            c.variant = 3;
+            c.probes_per_batch = P;
+            cands.push_back(c);
+        }
     const int threads[] = {256, 512, 1024};
     const int probes[] = {4, 8, 16, 32};
     for (int th : threads)

@@ -210,6 +210,8 @@ hipError_t launch_scatter_batch(const RadianceArgs* args_dev, const int* pbegin_
                                 int texel_bytes, hipStream_t stream);
 hipError_t launch_radiance_tiled(RadianceArgs a, int store, bool traced, const RadianceTune& t, int n_cu,
                                  hipStream_t stream);
+// Lane-refill variant: 64 rays per probe, P probes per wave (1..4).
+hipError_t launch_radiance_refill(const RadianceArgs& a, int store, int P, hipStream_t stream);
 size_t radiance_tiled_lds_bytes(const RadianceArgs& a, const RadianceTune& t);
 hipError_t launch_block_bits(const int16_t* blocks, int n_blocks, uint32_t* bits, hipStream_t stream);
 hipError_t launch_palette_bits(const uint8_t* palette, int first_slot, int n_slots, uint32_t* bits,

@@ -282,63 +282,65 @@ struct RayHit {
     int key;
 };
 
-// CastRay_fast: coarse 0.5-voxel march with empty-block skipping, then up to
-// five voxel-DDA refinement steps from origin + dir*(fraction - 1.5).
-LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_steps) {
-    RayHit r;
-    r.hit = 0;
-    r.voxel = 0;
-    r.fraction = 0.0f;
-    r.normal = {0.0f, 0.0f, 0.0f};
-    r.coarse = 0;
-    r.refine = 0;
-    r.key = 0;
-    const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    const f3 corner = {d.x > 0.0f ? 16.0f : 0.0f, d.y > 0.0f ? 16.0f : 0.0f, d.z > 0.0f ? 16.0f : 0.0f};
-    float fraction = 0.0f;
-    int it = 0;
-    int cur = 0;
-    while (true) {
-        // Iterations begun, counted before any lane can leave, so the count
-        // stays wave-uniform (scalar) on the GPU.
-        ++it;
-        fraction += 0.5f;
-        f3 pos = o + d * fraction;
-        const int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
-        // Block coordinate ivec3(p) / 16 (truncating) as trunc(p / 16): the
-        // same integer, p / 16 being exact, without the signed division.
-        const f3 ps = pos * 0.0625f;
-        const int bx = (int)ps.x, by = (int)ps.y, bz = (int)ps.z;
-        const bool inb =
-            (unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz;
-        const int blk = inb ? block_id(w, bx, by, bz) : 0;
-        cur = voxel_in_block(w, blk, ix, iy, iz);
-        r.coarse++;
-        if (blk == 0) {
-            // (float)(b * 16) + corner as trunc(p / 16) * 16 + corner: the same
-            // integers, exact in float, in one FMA per axis.  (Lanes whose b
-            // would overflow are outside the world and leave the loop below
-            // without a hit, so their fraction is never read.)
-            f3 box = {fmaf(truncf(ps.x), 16.0f, corner.x), fmaf(truncf(ps.y), 16.0f, corner.y),
-                      fmaf(truncf(ps.z), 16.0f, corner.z)};
-            f3 tmp = (-pos) * inv;
-            f3 bc = box * inv + tmp;
-            float f = fminf(bc.x, fminf(bc.y, bc.z));
-            f = fmaxf(f, 0.01f);
-            fraction += f;
-        }
-        if (cur != 0) {
-            r.hit = 1;
-            break;
-        }
-        // p outside [0, 16 W): a negative coordinate, or (p >= 0) a block
-        // coordinate past the far edge, since trunc(p / 16) < W <=> p < 16 W.
-        if (!inb || fminf(pos.x, fminf(pos.y, pos.z)) < 0.0f) break;
-        if (fraction >= max_dist) break;
-        if (max_steps > 0 && it >= max_steps) break;
-    }
-    if (!r.hit) return r;
+// A ray of CastRay_fast with the values its march reuses every sample.
+struct MarchRay {
+    f3 o, d, inv, corner;
+};
+LUMX_HD MarchRay march_ray(f3 o, f3 d) {
+    MarchRay m;
+    m.o = o;
+    m.d = d;
+    m.inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    m.corner = {d.x > 0.0f ? 16.0f : 0.0f, d.y > 0.0f ? 16.0f : 0.0f, d.z > 0.0f ? 16.0f : 0.0f};
+    return m;
+}
 
+// One coarse sample of CastRay_fast (radiance.comp:205-244): fraction += 0.5,
+// the voxel there, the empty-block skip.  Returns 1 on a hit, -1 when the ray
+// leaves (outside the world, past max_dist, or sample `it` reached
+// max_steps), 0 to go on.
+LUMX_HD int coarse_sample(const WorldView& w, const MarchRay& m, float max_dist, int max_steps, int it,
+                          float* fraction_io) {
+    float fraction = *fraction_io + 0.5f;
+    f3 pos = m.o + m.d * fraction;
+    const int ix = (int)pos.x, iy = (int)pos.y, iz = (int)pos.z;
+    // Block coordinate ivec3(p) / 16 (truncating) as trunc(p / 16): the
+    // same integer, p / 16 being exact, without the signed division.
+    const f3 ps = pos * 0.0625f;
+    const int bx = (int)ps.x, by = (int)ps.y, bz = (int)ps.z;
+    const bool inb = (unsigned)bx < (unsigned)w.wx && (unsigned)by < (unsigned)w.wy && (unsigned)bz < (unsigned)w.wz;
+    const int blk = inb ? block_id(w, bx, by, bz) : 0;
+    const int cur = voxel_in_block(w, blk, ix, iy, iz);
+    if (blk == 0) {
+        // (float)(b * 16) + corner as trunc(p / 16) * 16 + corner: the same
+        // integers, exact in float, in one FMA per axis.  (Lanes whose b
+        // would overflow are outside the world and leave below without a
+        // hit, so their fraction is never read.)
+        f3 box = {fmaf(truncf(ps.x), 16.0f, m.corner.x), fmaf(truncf(ps.y), 16.0f, m.corner.y),
+                  fmaf(truncf(ps.z), 16.0f, m.corner.z)};
+        f3 tmp = (-pos) * m.inv;
+        f3 bc = box * m.inv + tmp;
+        float f = fminf(bc.x, fminf(bc.y, bc.z));
+        f = fmaxf(f, 0.01f);
+        fraction += f;
+    }
+    *fraction_io = fraction;
+    if (cur != 0) return 1;
+    // p outside [0, 16 W): a negative coordinate, or (p >= 0) a block
+    // coordinate past the far edge, since trunc(p / 16) < W <=> p < 16 W.
+    if (!inb || fminf(pos.x, fminf(pos.y, pos.z)) < 0.0f) return -1;
+    if (fraction >= max_dist) return -1;
+    if (max_steps > 0 && it >= max_steps) return -1;
+    return 0;
+}
+
+// The refinement after a coarse hit at `fraction` (radiance.comp:246-288):
+// up to five voxel-DDA steps from origin + dir * (fraction - 1.5).
+LUMX_HD RayHit refine_hit(const WorldView& w, const MarchRay& m, float fraction) {
+    RayHit r;
+    r.hit = 1;
+    r.coarse = 0;
+    const f3 o = m.o, d = m.d, inv = m.inv;
     f3 bh = o + d * (fraction - 1.5f);
     int sx = d.x > 0.0f ? 1 : -1, sy = d.y > 0.0f ? 1 : -1, sz = d.z > 0.0f ? 1 : -1;
     f3 fl = {floorf(bh.x), floorf(bh.y), floorf(bh.z)};
@@ -369,6 +371,37 @@ LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_
     r.fraction = dot3(tf, st);
     r.voxel = cv;
     r.key = 1 + ((vx + 512) & 1023) + (((vy + 512) & 1023) << 10) + (((vz + 512) & 1023) << 20);
+    return r;
+}
+
+// CastRay_fast: coarse 0.5-voxel march with empty-block skipping, then up to
+// five voxel-DDA refinement steps from origin + dir*(fraction - 1.5).
+LUMX_HD RayHit cast_ray(const WorldView& w, f3 o, f3 d, float max_dist, int max_steps) {
+    const MarchRay m = march_ray(o, d);
+    float fraction = 0.0f;
+    int it = 0;
+    int coarse = 0;
+    int st;
+    while (true) {
+        // Samples begun, counted before any lane can leave, so the count
+        // stays wave-uniform (scalar) on the GPU.
+        ++it;
+        ++coarse;
+        st = coarse_sample(w, m, max_dist, max_steps, it, &fraction);
+        if (st != 0) break;
+    }
+    RayHit r;
+    if (st < 0) {
+        r.hit = 0;
+        r.voxel = 0;
+        r.fraction = 0.0f;
+        r.normal = {0.0f, 0.0f, 0.0f};
+        r.refine = 0;
+        r.key = 0;
+    } else {
+        r = refine_hit(w, m, fraction);
+    }
+    r.coarse = coarse;
     return r;
 }
 

@@ -75,10 +75,10 @@ __device__ __forceinline__ bool probe_coords(const RadianceArgs& a, int wg, int*
     return (unsigned)*px < (unsigned)w.wx && (unsigned)*py < (unsigned)w.wy && (unsigned)*pz < (unsigned)w.wz;
 }
 
-// Light gathered by ray r of probe wg at block (px, py, pz).
-template <int STORE, bool TRACED, bool OCC>
-__device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int r, int px, int py, int pz) {
-    const WorldView& w = a.world;
+// Ray r of probe wg at block (px, py, pz) (radiance.comp:370-381): origin
+// and direction.
+__device__ __forceinline__ void ray_setup(const RadianceArgs& a, int wg, int r, int px, int py, int pz, f3* pos_out,
+                                          f3* dir_out) {
     const int R = a.rays;
     const uint32_t t = (uint32_t)a.time_seed;
     const uint32_t gidx = (uint32_t)wg * (uint32_t)R + (uint32_t)r;
@@ -93,6 +93,38 @@ __device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int
     f3 pos = center + dir * 6.0f;
     pos = pos + (r1 - half) * 2.0f;
     if (r & 1) dir = -dir;
+    *pos_out = pos;
+    *dir_out = dir;
+}
+
+// trace_ray's shading of a cast ray from pos along dir (radiance.comp:316-347).
+template <int STORE>
+__device__ __forceinline__ f3 shade_ray(const RadianceArgs& a, const RayHit& h, f3 pos, f3 dir) {
+    const WorldView& w = a.world;
+    f3 light = {0.0f, 0.0f, 0.0f};
+    if (h.hit) {
+        MatF m = get_mat(w, h.voxel);
+        f3 prop = sample_radiance<STORE>(a.field_in, w.wx, w.wy, w.wz, pos + h.fraction * dir, h.normal);
+        light = light + (prop * m.color + m.color * m.emit);
+    } else {
+        float gl = -dot3(dir, radiance_light_dir());
+        if (gl > 0.95f) {
+            f3 sun = f3{0.9f, 0.9f, 0.6f} * 3.0f;
+            light = light + gl * sun;
+        }
+        f3 sky = f3{0.53f, 0.81f, 0.92f} * 0.1f;
+        light = light + sky;
+    }
+    return light;
+}
+
+// Light gathered by ray r of probe wg at block (px, py, pz).
+template <int STORE, bool TRACED, bool OCC>
+__device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int r, int px, int py, int pz) {
+    const WorldView& w = a.world;
+    const int R = a.rays;
+    f3 pos, dir;
+    ray_setup(a, wg, r, px, py, pz, &pos, &dir);
 
     RayHit h;
     if (OCC) {
@@ -110,20 +142,7 @@ __device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int
     } else {
         h = cast_ray(w, pos, dir, a.max_dist, a.max_steps);
     }
-    f3 light = {0.0f, 0.0f, 0.0f};
-    if (h.hit) {
-        MatF m = get_mat(w, h.voxel);
-        f3 prop = sample_radiance<STORE>(a.field_in, w.wx, w.wy, w.wz, pos + h.fraction * dir, h.normal);
-        light = light + (prop * m.color + m.color * m.emit);
-    } else {
-        float gl = -dot3(dir, radiance_light_dir());
-        if (gl > 0.95f) {
-            f3 sun = f3{0.9f, 0.9f, 0.6f} * 3.0f;
-            light = light + gl * sun;
-        }
-        f3 sky = f3{0.53f, 0.81f, 0.92f} * 0.1f;
-        light = light + sky;
-    }
+    const f3 light = shade_ray<STORE>(a, h, pos, dir);
     if (TRACED) {
         size_t ri = (size_t)wg * R + r;
         if (a.ray_keys) gst(a.ray_keys, ri, h.hit ? h.key : 0);
@@ -289,6 +308,124 @@ template <int STORE, bool TRACED, bool OCC, int K>
 __global__ void __launch_bounds__(1024) radiance_kernel(RadianceArgs a) {
     extern __shared__ f3 chunk_sum[];
     radiance_lane_body<STORE, TRACED, OCC, K>(a, blockIdx.x * blockDim.x + threadIdx.x, chunk_sum);
+}
+
+// Lane refill (tuning variant 3; 64 rays per probe).  A wave owns probes
+// p0 .. p0 + P - 1 as one pool of 64 P rays.  Every lane sets up its home
+// rays (ray `lane` of each probe) into LDS; then lanes march pool rays one
+// coarse sample per iteration, and a lane whose ray ends takes the next
+// unmarched ray of the pool (ballot of the finished lanes, rank by mbcnt),
+// so no lane idles while rays remain.  The coarse result of every ray (its
+// fraction at a hit, -1 for a miss) goes to LDS; each home lane then refines
+// and shades its rays and the probes are reduced and stored exactly as in
+// the one-ray-per-lane kernel, so results are bitwise the same.
+constexpr int kRefillFloats = 10;  // per pool ray: origin, direction, 1/direction, coarse fraction
+
+size_t radiance_refill_lds_bytes(int P) { return (size_t)4 * P * 64 * kRefillFloats * sizeof(float); }
+
+template <int STORE>
+__global__ void __launch_bounds__(256) radiance_refill_kernel(RadianceArgs a, int P) {
+    extern __shared__ float refill_pool[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int NR = P * 64;
+    float* rs = refill_pool + (size_t)wave * NR * kRefillFloats;
+    const int wg0 = (blockIdx.x * 4 + wave) * P;
+    const WorldView& w = a.world;
+    for (int j = 0; j < P; ++j) {
+        int px = 0, py = 0, pz = 0;
+        f3 pos = {-64.0f, -64.0f, -64.0f}, dir = {0.57735027f, 0.57735027f, 0.57735027f};
+        // A probe not dispatched or outside the world marches from outside it
+        // and leaves at its first sample; its light is not used.
+        if (probe_coords(a, wg0 + j, &px, &py, &pz)) ray_setup(a, wg0 + j, lane, px, py, pz, &pos, &dir);
+        const MarchRay m = march_ray(pos, dir);
+        float* r = rs + (j * 64 + lane) * kRefillFloats;
+        r[0] = m.o.x; r[1] = m.o.y; r[2] = m.o.z;
+        r[3] = m.d.x; r[4] = m.d.y; r[5] = m.d.z;
+        r[6] = m.inv.x; r[7] = m.inv.y; r[8] = m.inv.z;
+    }
+    __syncthreads();
+    auto load_ray = [&](int q) {
+        const float* r = rs + q * kRefillFloats;
+        MarchRay m;
+        m.o = {r[0], r[1], r[2]};
+        m.d = {r[3], r[4], r[5]};
+        m.inv = {r[6], r[7], r[8]};
+        m.corner = {m.d.x > 0.0f ? 16.0f : 0.0f, m.d.y > 0.0f ? 16.0f : 0.0f, m.d.z > 0.0f ? 16.0f : 0.0f};
+        return m;
+    };
+    int ray = lane;
+    MarchRay m = load_ray(ray);
+    float fraction = 0.0f;
+    int it = 0;
+    bool active = true;
+    int next = 64;  // wave-uniform: the next pool ray to hand out
+    while (true) {
+        int st = 0;
+        if (active) {
+            ++it;
+            st = coarse_sample(w, m, a.max_dist, a.max_steps, it, &fraction);
+        }
+        const bool done = active && st != 0;
+        if (done) rs[ray * kRefillFloats + 9] = st > 0 ? fraction : -1.0f;
+        const unsigned long long fin = __ballot(done);
+        if (fin) {
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(fin >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)fin, 0u));
+            if (done) {
+                const int q = next + rank;
+                if (q < NR) {
+                    ray = q;
+                    m = load_ray(q);
+                    fraction = 0.0f;
+                    it = 0;
+                } else {
+                    active = false;
+                }
+            }
+            next += __popcll(fin);
+        }
+        if (__ballot(active) == 0ull) break;
+    }
+    __syncthreads();
+    const f3 zero = {0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < P; ++j) {
+        int px = 0, py = 0, pz = 0;
+        const bool valid = probe_coords(a, wg0 + j, &px, &py, &pz);
+        const int q = j * 64 + lane;
+        const MarchRay hm = load_ray(q);
+        const float fr = rs[q * kRefillFloats + 9];
+        RayHit h;
+        if (fr >= 0.0f) {
+            h = refine_hit(w, hm, fr);
+        } else {
+            h.hit = 0;
+            h.voxel = 0;
+            h.fraction = 0.0f;
+            h.normal = zero;
+        }
+        const f3 light = valid ? shade_ray<STORE>(a, h, hm.o, hm.d) : zero;
+        const f3 total = butterfly(light, 64);
+        if (valid && lane == 0) store_probe<STORE>(a, wg0 + j, px, py, pz, total);
+    }
+}
+
+hipError_t launch_radiance_refill(const RadianceArgs& a, int store, int P, hipStream_t stream) {
+    if (a.rays != 64 || P < 1 || P > 4) return hipErrorInvalidValue;
+    if (a.n_probes <= 0) return hipSuccess;
+    const int grid = (a.n_probes + 4 * P - 1) / (4 * P);
+    const size_t lds = radiance_refill_lds_bytes(P);
+    switch (store) {
+        case STORE_F16:
+            hipLaunchKernelGGL(radiance_refill_kernel<STORE_F16>, dim3(grid), dim3(256), lds, stream, a, P);
+            break;
+        case STORE_BF16:
+            hipLaunchKernelGGL(radiance_refill_kernel<STORE_BF16>, dim3(grid), dim3(256), lds, stream, a, P);
+            break;
+        default:
+            hipLaunchKernelGGL(radiance_refill_kernel<STORE_F32>, dim3(grid), dim3(256), lds, stream, a, P);
+            break;
+    }
+    return hipGetLastError();
 }
 
 // Many worlds in one launch.  World w owns threads [begin[w], begin[w+1]),

@@ -481,3 +481,58 @@ def test_radiance_after_map_uses_dynamic_slots(gpu_ctx):
         for m in ms:
             m.close()
         w.close()
+
+
+@pytest.mark.parametrize("pool", [1, 2, 4])
+@pytest.mark.parametrize("size,max_steps", [((16, 16, 8), 0), ((48, 48, 16), 64), ((13, 11, 7), 0)])
+def test_radiance_refill_variant_bit_exact(gpu_ctx, pool, size, max_steps):
+    """The lane-refill kernel (variant 3: a wave marches `pool` probes' rays as
+    one pool, finished lanes taking the next ray) gives the oracle's field bit
+    for bit: a full update (13x11x7 leaves a partial last pool), then a
+    request list with size/shift, repeats and entries outside the world."""
+    import torch
+    blocks, pal, mats, field = make_case(size, 91)
+    wx, wy, wz = size
+    rng = np.random.default_rng(92)
+    flat = rng.integers(0, wx * wy * wz, size=257)
+    req = np.stack([flat % wx, (flat // wx) % wy, flat // (wx * wy), np.zeros(257, int)], 1).astype(np.int32)
+    req[3] = [wx, 0, 0, 0]
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    w.set_tuning(variant=3, probes_per_batch=pool)
+    try:
+        w.radiance_update(None, radiance_params(rays=64, time_seed=5, alpha=0.2, max_steps=max_steps))
+        w.radiance_update(torch.from_numpy(req).cuda(),
+                          radiance_params(rays=64, time_seed=6, size=2, shift=1, alpha=0.2, max_steps=max_steps))
+        torch.cuda.synchronize()
+        got = w.read_radiance()
+        wd = orc.WorldData(blocks, full_palette(w, pal), mats)
+    finally:
+        w.close()
+    ref = orc.radiance_update(wd, field, rays=64, time_seed=5, alpha=0.2, max_steps=max_steps)
+    ref = orc.radiance_update(wd, ref, requests=req, rays=64, time_seed=6, size=2, shift=1, alpha=0.2,
+                              max_steps=max_steps)
+    assert np.array_equal(got, ref)
+
+
+def test_radiance_refill_variant_fp16_and_rays(gpu_ctx):
+    """FP16 storage through the refill kernel equals the lane kernel's; a
+    ray count other than 64 is refused for it."""
+    import torch
+    from lum_amd._lib import RADIANCE_F16
+    size = (20, 16, 8)
+    blocks, pal, mats, field = make_case(size, 93)
+    outs = []
+    for variant in (0, 3):
+        w = gpu_world(gpu_ctx, blocks, pal, mats, field, fmt=RADIANCE_F16)
+        w.set_tuning(variant=variant, probes_per_batch=4)
+        try:
+            for f in range(2):
+                w.radiance_update(None, radiance_params(rays=64, time_seed=f, alpha=0.1))
+            torch.cuda.synchronize()
+            outs.append(w.read_radiance())
+            if variant == 3:
+                with pytest.raises(Exception):
+                    w.radiance_update(None, radiance_params(rays=32, time_seed=2, alpha=0.1))
+        finally:
+            w.close()
+    assert np.array_equal(outs[0], outs[1])
