@@ -953,7 +953,7 @@ def make_parser():
     ap.add_argument("--serving-worlds", type=int, default=200)
     ap.add_argument("--requests", type=int, default=0,
                     help="serving: timed requests (default 20 per step, 4000 at the default --steps)")
-    ap.add_argument("--max-batch", type=int, default=16, help="serving: requests per launch")
+    ap.add_argument("--max-batch", type=int, default=32, help="serving: requests per launch")
     ap.add_argument("--streams", type=int, default=4, help="serving: launches in flight")
     ap.add_argument("--hbm-budget-gb", type=float, default=0.0,
                     help="serving: HBM budget in GiB (default: --hbm-budget-frac of the worlds' device state)")

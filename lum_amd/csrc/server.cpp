@@ -68,6 +68,8 @@ struct ServedWorld {
     unsigned long long last_use = 0;
     size_t off = 0;  // arena offset while resident (with a budget)
     bool static_kept = false;  // its snapshot holds its static state, taken in this server
+    hipEvent_t ready = nullptr;  // recorded on the copy stream after its last restore
+    bool ready_pending = false;  // no launch has waited for that restore yet
 };
 
 struct Flight {
@@ -135,8 +137,6 @@ struct lumx_server {
     std::vector<hipEvent_t> free_events;
     int next_stream = 0;
     hipStream_t copy_stream = nullptr;  // evictions and restores
-    hipEvent_t copy_done = nullptr;     // recorded after the swaps a launch depends on
-    bool swaps_queued = false;          // swaps queued since copy_done was last recorded
     Arena arena;
 
     std::mutex mu;
@@ -243,7 +243,6 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
             s->arena.give(v.off, v.w->device_bytes);
             s->resident_bytes -= v.w->device_bytes;
             s->stats.evictions++;
-            s->swaps_queued = true;
         }
         if (!s->arena.take_at(start, need)) return false;
         off = start;
@@ -255,23 +254,45 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
     sw.off = off;
     s->resident_bytes += need;
     s->stats.restores++;
-    s->swaps_queued = true;
+    // Launches of this world wait for its restore (and, the copy stream being
+    // in order, for the evictions queued before it).
+    if (hipEventRecord(sw.ready, s->copy_stream) != hipSuccess) return false;
+    sw.ready_pending = true;
     return true;
 }
 
-// Launches on `st` start after every swap queued so far.
-static int order_after_swaps(lumx_server* s, hipStream_t st) {
-    if (!s->swaps_queued) return LUMX_OK;
-    HIP_TRY(hipEventRecord(s->copy_done, s->copy_stream));
-    HIP_TRY(hipStreamWaitEvent(st, s->copy_done, 0));
-    s->swaps_queued = false;
+// Launches on `st` of flight f start after its worlds' restores.
+static int order_after_restores(lumx_server* s, const Flight& f, hipStream_t st) {
+    for (int wi : f.worlds) {
+        ServedWorld& sw = s->worlds[wi];
+        if (!sw.ready_pending) continue;
+        HIP_TRY(hipStreamWaitEvent(st, sw.ready, 0));
+        sw.ready_pending = false;
+    }
     return LUMX_OK;
+}
+
+// While every launch slot is taken: restore the worlds of the next pending
+// requests (at most one launch's worth), evicting only idle worlds that none
+// of those requests names, so their launch finds them resident.
+static void prefetch(lumx_server* s) {
+    const int nw = (int)s->worlds.size();
+    std::vector<char> keep(nw, 0);
+    int n = 0;
+    for (auto it = s->pending.begin(); it != s->pending.end() && n < s->desc.max_batch; ++it, ++n)
+        keep[(*it)->spec.world] = 1;
+    n = 0;
+    for (auto it = s->pending.begin(); it != s->pending.end() && n < s->desc.max_batch; ++it, ++n) {
+        const int wi = (*it)->spec.world;
+        if (resident(s->worlds[wi].w)) continue;
+        if (!ensure_resident(s, wi, keep)) break;
+    }
 }
 
 // One request as its own single-world call (direct mode).
 static int launch_direct(lumx_server* s, Flight& f) {
     hipStream_t st = s->streams[f.stream];
-    int rc = order_after_swaps(s, st);
+    int rc = order_after_restores(s, f, st);
     if (rc) return rc;
     const lumx_request& r = f.reqs[0]->spec;
     lumx_world* w = s->worlds[f.worlds[0]].w;
@@ -287,7 +308,7 @@ static int launch_direct(lumx_server* s, Flight& f) {
 
 static int launch_flight(lumx_server* s, Flight& f) {
     hipStream_t st = s->streams[f.stream];
-    int rc0 = order_after_swaps(s, st);
+    int rc0 = order_after_restores(s, f, st);
     if (rc0) return rc0;
     std::vector<lumx_world*> rw, gw;
     std::vector<const int32_t*> rq;
@@ -454,6 +475,8 @@ static void worker_loop(lumx_server* s) {
             }
         }
         if (launched) continue;
+        if (!s->desc.direct && s->desc.hbm_budget && (int)s->flights.size() >= depth && !s->pending.empty())
+            prefetch(s);
         // 3. wait: poll in-flight events briefly, or sleep until a submission
         if (!s->flights.empty()) {
             lk.unlock();
@@ -487,14 +510,12 @@ int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_serve
         return rc;
     }
     const int n_streams = d->direct ? 1 : d->streams;
-    bool ok = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&s->copy_done, hipEventDisableTiming) == hipSuccess;
+    bool ok = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) == hipSuccess;
     if (ok && d->hbm_budget) {
         // The arena: every byte of world state the server may hold.
         void* p = nullptr;
         if (hipMalloc(&p, d->hbm_budget) != hipSuccess) {
             if (s->copy_stream) hipStreamDestroy(s->copy_stream);
-            if (s->copy_done) hipEventDestroy(s->copy_done);
             launcher_destroy(s->L);
             delete s;
             return fail(LUMX_ENOMEM, "HBM budget of %zu bytes could not be allocated", d->hbm_budget);
@@ -511,7 +532,6 @@ int lumx_server_create(lumx_context_t ctx, const lumx_server_desc* d, lumx_serve
     if (!ok) {
         for (hipStream_t t : s->streams) hipStreamDestroy(t);
         if (s->copy_stream) hipStreamDestroy(s->copy_stream);
-        if (s->copy_done) hipEventDestroy(s->copy_done);
         if (s->arena.base) hipFree(s->arena.base);
         launcher_destroy(s->L);
         delete s;
@@ -555,7 +575,8 @@ int lumx_server_destroy(lumx_server_t s) {
         hipFree(s->arena.base);
     }
     hipStreamDestroy(s->copy_stream);
-    hipEventDestroy(s->copy_done);
+    for (ServedWorld& sw : s->worlds)
+        if (sw.ready) hipEventDestroy(sw.ready);
     for (hipEvent_t e : s->free_events) hipEventDestroy(e);
     launcher_destroy(s->L);
     delete s;
@@ -574,13 +595,11 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
             return fail(LUMX_EINVAL, "world needs %zu bytes, more than the budget of %zu", w->device_bytes,
                         s->desc.hbm_budget);
     }
-    ServedWorld sw;
-    sw.w = w;
+    HIP_TRY(hipSetDevice(s->ctx->device));
     if (s->desc.hbm_budget && resident(w)) {
         // With a budget every world is served from the arena: a resident
         // world moves there while the arena has room (one device copy of its
         // block), else to the host, into its pinned snapshot.
-        HIP_TRY(hipSetDevice(s->ctx->device));
         HIP_TRY(hipDeviceSynchronize());  // work on the world may be queued on any stream
         // Its pinned snapshot now, so no eviction allocates while serving.
         int rc = world_snapshot_reserve(w, std::max(w->counter, w->prev_counter), s->copy_stream);
@@ -594,14 +613,17 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
             }
             if (w->dev_own) hipFree(w->dev_mem);
             world_rebase(w, s->arena.base + off, false);
-            sw.off = off;
         } else {
             rc = lumx_world_evict(w);
             if (rc) return rc;
         }
     }
+    ServedWorld sw;
+    sw.w = w;
+    if (resident(w) && !w->dev_own) sw.off = (size_t)((char*)w->dev_mem - s->arena.base);
     // A world on the host holds a full snapshot.
     sw.static_kept = s->desc.hbm_budget != 0 && !resident(w) && w->snap_static_valid;
+    HIP_TRY(hipEventCreateWithFlags(&sw.ready, hipEventDisableTiming));
     s->worlds.push_back(sw);
     if (resident(w)) s->resident_bytes += w->device_bytes;
     *id = (int)s->worlds.size() - 1;
