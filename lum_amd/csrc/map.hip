@@ -37,11 +37,6 @@ __device__ __forceinline__ void map_thread(const MapArgs& a, int gx, int gy, int
     const int iwx = (int)wvx, iwy = (int)wvy, iwz = (int)wvz;
 
     const int tbx = iwx / 16, tby = iwy / 16, tbz = iwz / 16;
-    int target = 0;
-    if ((unsigned)tbx < (unsigned)a.wx && (unsigned)tby < (unsigned)a.wy && (unsigned)tbz < (unsigned)a.wz)
-        target = a.blocks[tbx + a.wx * (tby + a.wy * tbz)];
-    if (target < a.static_count || target >= a.palette_slots) return;
-
     float mv[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -51,10 +46,15 @@ __device__ __forceinline__ void map_thread(const MapArgs& a, int gx, int gy, int
         mv[r] = ((p0 + p1) + p2) + a.m[r * 4 + 3];
     }
     const int imx = (int)mv[0], imy = (int)mv[1], imz = (int)mv[2];
-    if ((unsigned)imx >= (unsigned)a.mx || (unsigned)imy >= (unsigned)a.my || (unsigned)imz >= (unsigned)a.mz)
-        return;
-    const uint8_t v = a.model[imx + (size_t)a.mx * (imy + (size_t)a.my * imz)];
-    if (v == 0) return;
+    // The block id and the model voxel are independent loads: both are issued
+    // before either is tested.
+    int target = 0;
+    if ((unsigned)tbx < (unsigned)a.wx && (unsigned)tby < (unsigned)a.wy && (unsigned)tbz < (unsigned)a.wz)
+        target = a.blocks[tbx + a.wx * (tby + a.wy * tbz)];
+    uint8_t v = 0;
+    if ((unsigned)imx < (unsigned)a.mx && (unsigned)imy < (unsigned)a.my && (unsigned)imz < (unsigned)a.mz)
+        v = a.model[imx + (size_t)a.mx * (imy + (size_t)a.my * imz)];
+    if (target < a.static_count || target >= a.palette_slots || v == 0) return;
     const uint32_t off = ((uint32_t)target << 12) | (uint32_t)((iwx & 15) | ((iwy & 15) << 4) | ((iwz & 15) << 8));
     a.palette[off] = v;
 }
