@@ -277,9 +277,11 @@ def run_batched(args):
 
     params = [c.params for c in cases]
 
+    all_models = [c.models for c in cases]
+    all_placements = [c.placements for c in cases]
+
     def batched_pass():
-        for c in cases:
-            c.world.map_models_placed(c.models, c.placements)
+        batch.map_models(all_models, all_placements)
         batch.radiance(params)
         batch.glossy([c.gbuf[0] for c in cases], [c.gbuf[1] for c in cases], [c.gbuf[2] for c in cases],
                      [c.out for c in cases], 128.0, gmax)

@@ -334,6 +334,20 @@ int lumx_batch_autotune(lumx_batch_t b, const int32_t* const* requests_dev, cons
  * (0 = 256) and rays per lane 1, 2 or 4 (0 = 1; a launch whose smallest
  * rays per probe is below it uses 1).  Results do not depend on either. */
 int lumx_batch_set_shape(lumx_batch_t b, int block, int rays_per_lane);
+/* Map of every world of the batch: world i gets n_models[i] models
+ * models[i][0 ..] at placements[i][0 ..] (models[i] / placements[i] may be
+ * NULL when n_models[i] is 0).  Blockify runs on the host as for
+ * lumx_map_models_placed, then all worlds go up in one copy and are
+ * committed, copied and mapped by a handful of launches shared by every
+ * world.  Each world's result equals its own lumx_map_models_placed with the
+ * same arguments, bit for bit.  When world k's palette overflows, worlds
+ * 0 .. k-1 are mapped, world k is back at its origin, later worlds are
+ * untouched, and the overflow is returned (as k + 1 single-world calls would
+ * leave them).  Inside a graph capture the blockified worlds are those of
+ * capture time, uploaded again on every replay.  Batched maps set no
+ * per-world map timing (lumx_world_last_pass_ms). */
+int lumx_batch_map_models(lumx_batch_t b, const int* n_models, const lumx_model_t* const* models,
+                          const lumx_placement* const* placements, void* stream);
 /* Glossy pass of every world in one launch (one G-buffer and camera per world). */
 int lumx_batch_glossy(lumx_batch_t b, const uint8_t* const* matnorm_dev, const float* const* depth_dev,
                       const int* width, const int* height, const lumx_camera* cams, float max_dist, int max_steps,

@@ -140,6 +140,7 @@ _SIGS = {
     "lumx_batch_radiance": (_I, [_P, C.POINTER(_P), C.POINTER(C.c_int), C.POINTER(RadianceParams), _P]),
     "lumx_batch_glossy": (_I, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                C.POINTER(Camera), C.c_float, _I, C.POINTER(_P), _P]),
+    "lumx_batch_map_models": (_I, [_P, C.POINTER(C.c_int), C.POINTER(_P), C.POINTER(_P), _P]),
     "lumx_batch_set_frame": (_I, [_P, _I, _P]),
     "lumx_batch_advance_frame": (_I, [_P, _P]),
     "lumx_world_evict": (_I, [_P]),

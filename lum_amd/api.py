@@ -337,6 +337,26 @@ class Batch:
     def set_shape(self, block=0, rays_per_lane=0):
         check(self.lib.lumx_batch_set_shape(self.handle, int(block), int(rays_per_lane)))
 
+    def map_models(self, models, placements):
+        """models[i], placements[i]: world i's models and placements (lists;
+        empty = the world's origin).  Equal to World.map_models_placed on
+        each world in order."""
+        n = len(self.worlds)
+        keep = []
+        hs = (C.c_void_p * n)()
+        ps = (C.c_void_p * n)()
+        cnt = (C.c_int * n)()
+        for i in range(n):
+            k = len(models[i])
+            cnt[i] = k
+            if k:
+                h = (C.c_void_p * k)(*[m.handle.value for m in models[i]])
+                p = (Placement * k)(*placements[i])
+                keep += [h, p]
+                hs[i] = C.cast(h, C.c_void_p)
+                ps[i] = C.cast(p, C.c_void_p)
+        check(self.lib.lumx_batch_map_models(self.handle, cnt, hs, ps, _stream()))
+
     def glossy(self, matnorms, depths, cams, outs, max_dist=128.0, max_steps=0):
         n = len(self.worlds)
         mn = (C.c_void_p * n)(*[m.data_ptr() for m in matnorms])

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "host_internal.hpp"
@@ -48,6 +49,12 @@ struct Launcher {
     std::vector<DescSlot> capture_pool;  // pre-allocated, handed to recorded passes
     std::vector<DescSlot> captured;      // owned by recorded graphs until destroy
     int ring_next = 0;
+    // Map staging: blocks, copy lists and launch descriptors of a batched
+    // map, sized on demand.  Eager maps alternate between two; a map recorded
+    // into a graph gets its own, kept until destroy.
+    DescSlot map_ring[2];
+    int map_next = 0;
+    std::vector<DescSlot> map_captured;
     int* frame_dev = nullptr;
     int block = 0;          // radiance workgroup size (0 = 256)
     int rays_per_lane = 0;  // rays per thread of the radiance launches (0 = 1)
@@ -88,6 +95,8 @@ void launcher_destroy(Launcher* L) {
     for (DescSlot& s : L->ring) free_slot(s);
     for (DescSlot& s : L->capture_pool) free_slot(s);
     for (DescSlot& s : L->captured) free_slot(s);
+    for (DescSlot& s : L->map_ring) free_slot(s);
+    for (DescSlot& s : L->map_captured) free_slot(s);
     hipFree(L->frame_dev);
     delete L;
 }
@@ -220,6 +229,163 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
             if (!args[i].staged) worlds[i]->cur ^= 1;  // full updates wrote the other field
     }
     return release_slot(slot, s);
+}
+
+// Batched map: per world the host half of lumx_map_models_placed
+// (map_prepare), then for all worlds together one upload of the blockified
+// worlds, copy lists and launch descriptors, one commit launch (blocks,
+// empty-block bits, copy lists, counters), one exec_copies launch, the map
+// launches (group k of every world in launch k: groups of one world stay in
+// order, different worlds write disjoint memory) and one palette-bits launch.
+int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_models,
+                 const lumx_model_t* const* models, const lumx_placement* const* pl, hipStream_t s) {
+    if (n <= 0) return LUMX_OK;
+    if (!worlds || !n_models) return fail(LUMX_EINVAL, "null argument");
+    for (int i = 0; i < n; ++i) {
+        if (!worlds[i] || worlds[i]->ctx != L->ctx) return fail(LUMX_EINVAL, "world %d invalid", i);
+        const bool some = n_models[i] > 0;
+        int rc = map_check(worlds[i], n_models[i], some && models ? models[i] : nullptr,
+                           some && pl ? pl[i] : nullptr);
+        if (rc) return rc;
+        for (int j = 0; j < i; ++j)
+            if (worlds[j] == worlds[i]) return fail(LUMX_EINVAL, "world %d appears twice in one map", i);
+    }
+    HIP_TRY(hipSetDevice(L->ctx->device));
+    const bool capturing = stream_capturing(s);
+    // Launch plans (they do not depend on blockify); launch k takes group k
+    // of every world that has one.
+    std::vector<std::vector<MapArgs>> args(n);
+    std::vector<std::vector<int>> ends(n);
+    for (int i = 0; i < n; ++i) {
+        int rc = map_plan(worlds[i], n_models[i], n_models[i] > 0 ? models[i] : nullptr,
+                          n_models[i] > 0 ? pl[i] : nullptr, &args[i], &ends[i]);
+        if (rc) return rc;
+    }
+    // Host blockify, world by world.  When world k fails the worlds before it
+    // are still committed, as a sequence of single-world maps would leave them.
+    int failed_rc = LUMX_OK;
+    int m = n;
+    for (int i = 0; i < n; ++i) {
+        int rc = map_prepare(worlds[i], n_models[i], n_models[i] > 0 ? pl[i] : nullptr);
+        if (rc) {
+            failed_rc = rc;
+            m = i;
+            break;
+        }
+    }
+    std::string saved_error;
+    if (failed_rc) saved_error = lumx_last_error();
+    if (m == 0) return failed_rc;
+    size_t n_launches = 0;
+    for (int i = 0; i < m; ++i) n_launches = std::max(n_launches, ends[i].size());
+    // Staging layout.
+    std::vector<MapWorldDesc> desc(m);
+    size_t off = round_up((size_t)m * sizeof(MapWorldDesc), 256);
+    std::vector<size_t> launch_off(n_launches), launch_begin_off(n_launches);
+    std::vector<int> launch_count(n_launches, 0);
+    for (size_t k = 0; k < n_launches; ++k) {
+        for (int i = 0; i < m; ++i)
+            if (k < ends[i].size()) launch_count[k] += ends[i][k] - (k ? ends[i][k - 1] : 0);
+        launch_off[k] = off;
+        off += round_up((size_t)launch_count[k] * sizeof(MapArgs), 256);
+        launch_begin_off[k] = off;
+        off += round_up(((size_t)launch_count[k] + 1) * sizeof(int), 256);
+    }
+    long long groups = 0, slots = 0;
+    for (int i = 0; i < m; ++i) {
+        lumx_world* w = worlds[i];
+        MapWorldDesc& d = desc[i];
+        const int st = w->desc.static_blocks;
+        const int hi = std::max(w->counter, w->prev_counter);
+        for (int k = w->counter; k < hi; ++k) w->copy_src[k] = 0;
+        d.blocks = w->blocks;
+        d.copy_src = w->copy_src_dev;
+        d.state = w->blockify_state;
+        d.empty_bits = w->empty_bits;
+        d.palette = w->palette;
+        d.pal_bits = w->pal_bits;
+        d.n_blocks = w->n_blocks;
+        d.st = st;
+        d.hi = std::max(hi, st);
+        d.counter = w->counter;
+        d.stage_blocks = off;
+        off += round_up((size_t)w->n_blocks * 2, 256);
+        d.stage_copy = off;
+        off += round_up((size_t)(d.hi - st) * 4, 256);
+        d.group_begin = (int)groups;
+        groups += (map_commit_elements(d) + 255) / 256;
+        d.slot_begin = (int)slots;
+        slots += d.hi - st;
+    }
+    if (groups > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batched map too large");
+    const size_t need = off;
+    // The staging this map writes.
+    DescSlot* slot = nullptr;
+    if (capturing) {
+        DescSlot fresh;
+        int rc = alloc_slot(fresh, need);
+        if (rc) return rc;
+        L->map_captured.push_back(fresh);
+        slot = &L->map_captured.back();
+    } else {
+        slot = &L->map_ring[L->map_next];
+        L->map_next ^= 1;
+        if (slot->pending) HIP_TRY(hipEventSynchronize(slot->done));
+        slot->pending = false;
+        if (slot->bytes < need) {
+            free_slot(*slot);
+            int rc = alloc_slot(*slot, round_up(need + need / 4, 1 << 16));
+            if (rc) return rc;
+        }
+    }
+    char* h = (char*)slot->host;
+    char* dv = (char*)slot->dev;
+    std::memcpy(h, desc.data(), (size_t)m * sizeof(MapWorldDesc));
+    std::vector<int> total_groups(n_launches, 0);
+    for (size_t k = 0; k < n_launches; ++k) {
+        MapArgs* a = (MapArgs*)(h + launch_off[k]);
+        int* begin = (int*)(h + launch_begin_off[k]);
+        int c = 0;
+        long long g = 0;
+        for (int i = 0; i < m; ++i) {
+            if (k >= ends[i].size()) continue;
+            for (int j = k ? ends[i][k - 1] : 0; j < ends[i][k]; ++j) {
+                const MapArgs& x = args[i][j];
+                a[c] = x;
+                begin[c] = (int)g;
+                g += (long long)x.tiles_x * x.tiles_y * (x.nz / 4);
+                ++c;
+            }
+        }
+        if (g > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batched map launch too large");
+        begin[c] = (int)g;
+        total_groups[k] = (int)g;
+    }
+    for (int i = 0; i < m; ++i) {
+        const lumx_world* w = worlds[i];
+        std::memcpy(h + desc[i].stage_blocks, w->current.data(), (size_t)w->n_blocks * 2);
+        if (desc[i].hi > desc[i].st)
+            std::memcpy(h + desc[i].stage_copy, w->copy_src.data() + desc[i].st,
+                        (size_t)(desc[i].hi - desc[i].st) * 4);
+    }
+    HIP_TRY(hipMemcpyAsync(dv, h, need, hipMemcpyHostToDevice, s));
+    const MapWorldDesc* d_dev = (const MapWorldDesc*)dv;
+    HIP_TRY(launch_map_commit_batch(d_dev, m, (int)groups, dv, s));
+    HIP_TRY(launch_map_copies_batch(d_dev, m, (int)slots, s));
+    for (size_t k = 0; k < n_launches; ++k)
+        HIP_TRY(launch_map_batch((const MapArgs*)(dv + launch_off[k]), (const int*)(dv + launch_begin_off[k]),
+                                 launch_count[k], total_groups[k], s));
+    HIP_TRY(launch_palette_bits_batch(d_dev, m, (int)slots, s));
+    for (int i = 0; i < m; ++i) {
+        worlds[i]->prev_counter = worlds[i]->counter;
+        worlds[i]->ev_valid[1] = false;  // batched maps are not timed per world
+    }
+    if (!capturing) {
+        HIP_TRY(hipEventRecord(slot->done, s));
+        slot->pending = true;
+    }
+    if (failed_rc) return fail(failed_rc, "%s", saved_error.c_str());
+    return LUMX_OK;
 }
 
 int launcher_glossy(Launcher* L, lumx_world* const* worlds, int n, const uint8_t* const* matnorm,
@@ -391,6 +557,13 @@ int lumx_batch_set_shape(lumx_batch_t b, int block, int rays_per_lane) {
     if (rc) return rc;
     clear_error();
     return LUMX_OK;
+}
+
+int lumx_batch_map_models(lumx_batch_t b, const int* n_models, const lumx_model_t* const* models,
+                          const lumx_placement* const* placements, void* stream) {
+    if (!b || !n_models) return fail(LUMX_EINVAL, "null argument");
+    return launcher_map(b->L, b->worlds.data(), (int)b->worlds.size(), n_models, models, placements,
+                        (hipStream_t)stream);
 }
 
 int lumx_batch_glossy(lumx_batch_t b, const uint8_t* const* matnorm, const float* const* depth, const int* width,

@@ -985,8 +985,10 @@ static int blockify(lumx_world* w, const lumx_placement& p) {
 // below 0 on some axis writes through aliased coordinates into block 0
 // (map.hip), outside its box: it gets a layer of its own, mapped as one launch
 // per sign class in increasing class order.
-static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
-                         hipStream_t s) {
+extern "C++" {
+template <class Emit>
+static int plan_map_launches(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                             Emit&& emit) {
     const int st = w->desc.static_blocks;
     SmallBuf<MapArgs, kNoAllocModels> args(std::max(n, 1));
     SmallBuf<int, kNoAllocModels> layer(std::max(n, 1), 0);
@@ -1033,48 +1035,66 @@ static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const
         }
     }
     const int n_layers = n ? *std::max_element(&layer[0], &layer[0] + n) + 1 : 0;
+    SmallBuf<MapArgs, kNoAllocModels> group(std::max(n, 1));
     for (int l = 0; l < n_layers; ++l) {
+        int g = 0;
+        for (int i = 0; i < n; ++i) {
+            if (layer[i] != l) continue;
+            const MapArgs& a = args[i];
+            const long long groups = (long long)a.tiles_x * a.tiles_y * (a.nz / 4);
+            if (groups <= 0) continue;
+            if (negative[i]) {
+                // Alone in its layer: one launch per sign class present.
+                const int lo[3] = {a.shift_x, a.shift_y, a.shift_z};
+                const int hi[3] = {a.shift_x + a.nx, a.shift_y + a.ny, a.shift_z + a.nz};
+                for (int cls = 0; cls < 8; ++cls) {
+                    bool present = true;
+                    for (int k = 0; k < 3; ++k) present &= (cls >> k & 1) ? hi[k] > 0 : lo[k] < 0;
+                    if (!present) continue;
+                    MapArgs one = a;
+                    one.cls = cls;
+                    int rc = emit(&one, 1);
+                    if (rc) return rc;
+                }
+                continue;
+            }
+            group[g++] = a;
+        }
+        if (g > 0) {
+            int rc = emit(&group[0], g);
+            if (rc) return rc;
+        }
+    }
+    return LUMX_OK;
+}
+}  // extern "C++"
+
+// The launches of plan_map_launches in order, up to kMapGroup models per
+// launch (descriptors as the kernel argument).  Allocates nothing.
+static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                         hipStream_t s) {
+    return plan_map_launches(w, n, models, pl, [&](const MapArgs* group, int count) -> int {
         MapGroup g;
         g.n = 0;
         g.begin[0] = 0;
-        for (int i = 0; i <= n; ++i) {
-            const bool take = i < n && layer[i] == l;
-            if (take) {
-                const MapArgs& a = args[i];
+        for (int k = 0; k <= count; ++k) {
+            if (k < count) {
+                const MapArgs& a = group[k];
                 const long long groups = (long long)a.tiles_x * a.tiles_y * (a.nz / 4);
-                if (groups <= 0) continue;
                 if ((long long)g.begin[g.n] + groups > (1ll << 31) - 1)
-                    return fail(LUMX_EINVAL, "model %d dispatch too large", i);
-                if (negative[i]) {
-                    // Alone in its layer: one launch per sign class present.
-                    const int lo[3] = {a.shift_x, a.shift_y, a.shift_z};
-                    const int hi[3] = {a.shift_x + a.nx, a.shift_y + a.ny, a.shift_z + a.nz};
-                    for (int cls = 0; cls < 8; ++cls) {
-                        bool present = true;
-                        for (int k = 0; k < 3; ++k) present &= (cls >> k & 1) ? hi[k] > 0 : lo[k] < 0;
-                        if (!present) continue;
-                        MapGroup one;
-                        one.n = 1;
-                        one.m[0] = a;
-                        one.m[0].cls = cls;
-                        one.begin[0] = 0;
-                        one.begin[1] = (int)groups;
-                        HIP_TRY(launch_map_group(one, s));
-                    }
-                    continue;
-                }
+                    return fail(LUMX_EINVAL, "map launch too large");
                 g.m[g.n] = a;
                 g.begin[g.n + 1] = g.begin[g.n] + (int)groups;
                 g.n++;
             }
-            if (g.n == kMapGroup || (i == n && g.n > 0)) {
+            if (g.n == kMapGroup || (k == count && g.n > 0)) {
                 HIP_TRY(launch_map_group(g, s));
                 g.n = 0;
                 g.begin[0] = 0;
             }
         }
-    }
-    return LUMX_OK;
+        return LUMX_OK;
+    });
 }
 
 // Block box of blockify_mesh for a placed model (internal_render.cpp:1032-1037).
@@ -1250,18 +1270,22 @@ int lumx_map_models_device_noalloc(lumx_world_t w, int n, const lumx_model_t* mo
     return map_device(w, n, models, pl, stream, true);
 }
 
-int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
-                           void* stream) {
+extern "C++" {
+int lumx::map_check(const lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl) {
     if (!w || n < 0 || (n > 0 && (!models || !pl))) return fail(LUMX_EINVAL, "bad argument");
     for (int i = 0; i < n; ++i)
         if (!models[i] || models[i]->ctx != w->ctx) return fail(LUMX_EINVAL, "model %d invalid", i);
     REQUIRE_RESIDENT(w);
+    return LUMX_OK;
+}
+
+// start_blockify + blockify_mesh of each model (internal_render.cpp:1006-1092)
+// on the host world; on failure the host world is back at its origin.
+int lumx::map_prepare(lumx_world* w, int n, const lumx_placement* pl) {
     HIP_TRY(hipSetDevice(w->ctx->device));
     int rc0 = sync_host_state(w);
     if (rc0) return rc0;
     w->dev_overflow = false;
-    hipStream_t s = pick_stream(w->ctx, stream);
-    // start_blockify: current = origin, counter = static.
     w->current = w->origin;
     w->counter = w->desc.static_blocks;
     for (int i = 0; i < n; ++i) {
@@ -1272,6 +1296,26 @@ int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, co
             return rc;
         }
     }
+    return LUMX_OK;
+}
+
+int lumx::map_plan(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                   std::vector<MapArgs>* args, std::vector<int>* ends) {
+    return plan_map_launches(w, n, models, pl, [&](const MapArgs* group, int count) -> int {
+        args->insert(args->end(), group, group + count);
+        ends->push_back((int)args->size());
+        return LUMX_OK;
+    });
+}
+}  // extern "C++"
+
+int lumx_map_models_placed(lumx_world_t w, int n, const lumx_model_t* models, const lumx_placement* pl,
+                           void* stream) {
+    int rc0 = map_check(w, n, models, pl);
+    if (rc0) return rc0;
+    rc0 = map_prepare(w, n, pl);
+    if (rc0) return rc0;
+    hipStream_t s = pick_stream(w->ctx, stream);
     // end_blockify + exec_copies + map.
     const bool capturing = stream_capturing(s);
     if (!capturing) HIP_TRY(hipEventRecord(w->ev[1][0], s));

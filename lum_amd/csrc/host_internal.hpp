@@ -158,6 +158,17 @@ int world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s);
 // The world's device parts now live in `block` (a copy of its old block).
 void world_rebase(lumx_world* w, void* block, bool own);
 
+// Host-blockified maps (lumx_map_models_placed and its batched form):
+//   map_check    arguments of one world's map;
+//   map_prepare  start_blockify and blockify of each model on the host world
+//                (back at its origin on failure);
+//   map_plan     the world's map launches in order: group k is
+//                args[ends[k - 1] .. ends[k] - 1] (appended).
+int map_check(const lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl);
+int map_prepare(lumx_world* w, int n, const lumx_placement* pl);
+int map_plan(lumx_world* w, int n, const lumx_model_t* models, const lumx_placement* pl, std::vector<MapArgs>* args,
+             std::vector<int>* ends);
+
 // Validation shared by single-world and batched radiance updates.
 int radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p);
 // The checks of radiance_check that do not need the world.
@@ -180,6 +191,10 @@ int launcher_set_frame(Launcher* L, int value, int add, hipStream_t s);
 int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32_t* const* requests,
                       const int* n_requests, const lumx_radiance_params* params, bool use_frame, hipStream_t s,
                       bool served, bool commit = true);
+// Host-blockified maps of many worlds in one upload and a few launches; the
+// same result as lumx_map_models_placed on each world in order.
+int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_models,
+                 const lumx_model_t* const* models, const lumx_placement* const* pl, hipStream_t s);
 int launcher_glossy(Launcher* L, lumx_world* const* worlds, int n, const uint8_t* const* matnorm,
                     const float* const* depth, const int* width, const int* height, const lumx_camera* cams,
                     const float* max_dist, const int* max_steps, float* const* out, hipStream_t s);

@@ -22,6 +22,8 @@
 // copy_kernel restates the transfer part of exec_copies
 // (src/renderer/src/internal_render.cpp:1375-1476): every slot allocated by
 // blockify receives a copy of its static source block, or zeros.
+#include <algorithm>
+
 #include "lumx_internal.hpp"
 
 namespace lumx {
@@ -66,12 +68,8 @@ __device__ __forceinline__ void map_thread(const MapArgs& a, int gx, int gy, int
 // fastest, so a row of 16 threads stores into one 16-byte palette row); the
 // tile is found by 32-bit divisions of the workgroup index, once per
 // workgroup on the scalar unit.
-__global__ void __launch_bounds__(256) map_group_kernel(MapGroup g) {
-    const int b = blockIdx.x;
-    int i = 0;
-    while (i + 1 < g.n && g.begin[i + 1] <= b) ++i;
-    const MapArgs& a = g.m[i];
-    const uint32_t t = (uint32_t)(b - g.begin[i]);
+// Tile t of model a's dispatch box (16 x 4 x 4 threads, this thread's part).
+__device__ __forceinline__ void map_tile(const MapArgs& a, uint32_t t) {
     const uint32_t r = fast_div(t, a.div_tiles_x);
     const uint32_t bx = t - r * (uint32_t)a.tiles_x;
     const uint32_t bz = fast_div(r, a.div_tiles_y);
@@ -81,6 +79,107 @@ __global__ void __launch_bounds__(256) map_group_kernel(MapGroup g) {
     const int gz = (int)(bz * 4u + (threadIdx.x >> 6));
     if (gx >= a.nx) return;
     map_thread(a, gx, gy, gz);
+}
+
+__global__ void __launch_bounds__(256) map_group_kernel(MapGroup g) {
+    const int b = blockIdx.x;
+    int i = 0;
+    while (i + 1 < g.n && g.begin[i + 1] <= b) ++i;
+    map_tile(g.m[i], (uint32_t)(b - g.begin[i]));
+}
+
+// Last index i of begin[0 .. n - 1] (non-decreasing) with begin[i] <= x, on
+// the scalar unit when x is wave-uniform.
+template <class T>
+__device__ __forceinline__ int find_range(const T* begin, int n, T x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Models of many worlds, descriptors in memory; the same per-thread work as
+// map_group_kernel.
+__global__ void __launch_bounds__(256) map_batch_kernel(const MapArgs* args, const int* begin, int n) {
+    const int b = (int)blockIdx.x;
+    const int i = __builtin_amdgcn_readfirstlane(find_range(begin, n, b));
+    const MapArgs a = args[i];
+    map_tile(a, (uint32_t)(b - begin[i]));
+}
+
+// Elements of a world in the commit launch: its blocks, its empty-block bit
+// words, its copy list (slots st .. hi - 1), its counter.
+__host__ __device__ inline int commit_elements(int n_blocks, int st, int hi) {
+    return n_blocks + (n_blocks + 31) / 32 + (hi - st) + 1;
+}
+
+int map_commit_elements(const MapWorldDesc& d) { return commit_elements(d.n_blocks, d.st, d.hi); }
+
+// Last index i of d[0 .. n - 1] whose range starts at or before workgroup b
+// (ranges of empty worlds start where the next one does).
+template <int MapWorldDesc::*Begin>
+__device__ __forceinline__ int world_of(const MapWorldDesc* d, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].*Begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ void __launch_bounds__(256) map_commit_batch_kernel(const MapWorldDesc* d, int n, const char* stage) {
+    const MapWorldDesc& w = d[world_of<&MapWorldDesc::group_begin>(d, n, (int)blockIdx.x)];
+    int j = ((int)blockIdx.x - w.group_begin) * 256 + (int)threadIdx.x;
+    const int16_t* sb = (const int16_t*)(stage + w.stage_blocks);
+    if (j < w.n_blocks) {
+        w.blocks[j] = sb[j];
+        return;
+    }
+    j -= w.n_blocks;
+    const int ebw = (w.n_blocks + 31) / 32;
+    if (j < ebw) {
+        const int base = j * 32;
+        uint32_t v = 0;
+        for (int k = 0; k < 32 && base + k < w.n_blocks; ++k) v |= (sb[base + k] == 0 ? 1u : 0u) << k;
+        w.empty_bits[j] = v;
+        return;
+    }
+    j -= ebw;
+    if (j < w.hi - w.st) {
+        w.copy_src[w.st + j] = ((const int32_t*)(stage + w.stage_copy))[j];
+        return;
+    }
+    if (j == w.hi - w.st) w.state[0] = w.counter;
+}
+
+// One workgroup of 256 threads per slot, 16 B per thread: exec_copies.
+__global__ void __launch_bounds__(256) map_copies_batch_kernel(const MapWorldDesc* d, int n) {
+    const int b = (int)blockIdx.x;
+    const MapWorldDesc& w = d[world_of<&MapWorldDesc::slot_begin>(d, n, b)];
+    const int slot = w.st + (b - w.slot_begin);
+    const int src = w.copy_src[slot];
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (src != 0) v = ((const uint4*)(w.palette + ((size_t)src << 12)))[threadIdx.x];
+    ((uint4*)(w.palette + ((size_t)slot << 12)))[threadIdx.x] = v;
+}
+
+// One workgroup of 128 threads per slot: the slot's voxel-occupancy words.
+__global__ void __launch_bounds__(128) palette_bits_batch_kernel(const MapWorldDesc* d, int n) {
+    const int b = (int)blockIdx.x;
+    const MapWorldDesc& w = d[world_of<&MapWorldDesc::slot_begin>(d, n, b)];
+    const int slot = w.st + (b - w.slot_begin);
+    const uint8_t* src = w.palette + ((size_t)slot << 12) + threadIdx.x * 32;
+    const uint4 a0 = ((const uint4*)src)[0];
+    const uint4 a1 = ((const uint4*)src)[1];
+    const uint32_t words[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v |= (((words[k >> 2] >> ((k & 3) * 8)) & 0xffu) != 0 ? 1u : 0u) << k;
+    w.pal_bits[((size_t)slot << 7) + threadIdx.x] = v;
 }
 
 // One workgroup of 256 threads per slot, 16 B per thread.
@@ -96,6 +195,32 @@ __global__ void __launch_bounds__(256) copy_kernel(CopyArgs a) {
 hipError_t launch_map_group(const MapGroup& g, hipStream_t stream) {
     if (g.n <= 0 || g.begin[g.n] <= 0) return hipSuccess;
     hipLaunchKernelGGL(map_group_kernel, dim3(g.begin[g.n]), dim3(256), 0, stream, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_batch(const MapArgs* args_dev, const int* begin_dev, int n_models, int total_groups,
+                            hipStream_t stream) {
+    if (n_models <= 0 || total_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(map_batch_kernel, dim3(total_groups), dim3(256), 0, stream, args_dev, begin_dev, n_models);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_commit_batch(const MapWorldDesc* d_dev, int n, int total_groups, const char* stage_dev,
+                                   hipStream_t stream) {
+    if (n <= 0 || total_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(map_commit_batch_kernel, dim3(total_groups), dim3(256), 0, stream, d_dev, n, stage_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_copies_batch(const MapWorldDesc* d_dev, int n, int total_slots, hipStream_t stream) {
+    if (n <= 0 || total_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(map_copies_batch_kernel, dim3(total_slots), dim3(256), 0, stream, d_dev, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_palette_bits_batch(const MapWorldDesc* d_dev, int n, int total_slots, hipStream_t stream) {
+    if (n <= 0 || total_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(palette_bits_batch_kernel, dim3(total_slots), dim3(128), 0, stream, d_dev, n);
     return hipGetLastError();
 }
 
