@@ -141,6 +141,12 @@ typedef struct {
     int rays_per_lane;      /* variant 0: rays traced one after another by one thread (0 or 1, 2, 4);
                                used when rays per probe is a power of two >= it, else 1.  The
                                results do not depend on it. */
+    int compact;            /* subset updates of variants 0 and 2: run the requested probes once
+                               each in texel order (claims, then an ordered compaction of the
+                               claimed texels) instead of in list order.  0 = when it pays (at
+                               least 2048 probes and a sixteenth of the world, and a field larger
+                               than 4 MiB), 1 = always, -1 = never.  The results do not depend
+                               on it.  Batched and served launches compact when it pays. */
 } lumx_radiance_tuning;
 
 /* ---- context / errors ---- */

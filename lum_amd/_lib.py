@@ -51,7 +51,7 @@ class Placement(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("variant", C.c_int), ("threads", C.c_int), ("probes_per_batch", C.c_int),
                 ("lds_block_bits", C.c_int), ("lds_palette_bits", C.c_int), ("wg_per_cu", C.c_int),
-                ("rays_per_lane", C.c_int)]
+                ("rays_per_lane", C.c_int), ("compact", C.c_int)]
 
 
 class Camera(C.Structure):

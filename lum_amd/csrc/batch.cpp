@@ -107,7 +107,7 @@ int launcher_create(lumx_context* ctx, int max_worlds, int ring_slots, Launcher*
     L->ctx = ctx;
     L->max_worlds = max_worlds;
     L->slot_bytes = round_up((size_t)max_worlds * record_bytes(), 256) +
-                    2 * round_up(((size_t)max_worlds + 1) * sizeof(int), 256);
+                    3 * round_up(((size_t)max_worlds + 1) * sizeof(int), 256);
     L->ring.resize(ring_slots);
     L->capture_pool.resize(8);
     L->captured.reserve(64);
@@ -200,15 +200,24 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
     RadianceArgs* args = (RadianceArgs*)slot->host;
     int* begin = (int*)((char*)slot->host + args_bytes);
     int* pbegin = (int*)((char*)slot->host + args_bytes + begin_bytes);  // probe ranges of staged worlds
-    long long total = 0, total_staged = 0;
+    int* tbegin = (int*)((char*)slot->host + args_bytes + 2 * begin_bytes);  // tile ranges of compacted ones
+    long long total = 0, total_staged = 0, total_tiles = 0;
     for (int i = 0; i < n; ++i) {
         lumx_world* w = worlds[i];
         const int32_t* req = requests ? requests[i] : nullptr;
         int nr = req ? n_requests[i] : 0;
         rc = radiance_args(w, req, nr, &params[i], &args[i]);
         if (rc) return rc;
-        // The scatter that applies a subset update consumes (and resets) the claims.
-        if (args[i].staged && commit) args[i].claim = w->upd_claim;
+        // A subset update claims its probes before the launch; the scatter
+        // that applies it consumes (and resets) the claims.  A large one runs
+        // over its claimed probes in texel order (compacted).
+        if (args[i].staged && commit) {
+            args[i].claim = w->upd_claim;
+            if (compact_pays(args[i].n_probes, w->n_blocks)) compact_args(w, &args[i]);
+        }
+        tbegin[i] = (int)total_tiles;
+        total_tiles += args[i].compact ? args[i].n_tiles : 0;
+        if (total_tiles > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batch too large");
         begin[i] = (int)total;
         total += round_up((size_t)args[i].n_probes * (args[i].rays / K), (size_t)block);
         if (total > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batch too large");
@@ -216,8 +225,15 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
         if (args[i].staged) total_staged += args[i].n_probes;
     }
     pbegin[n] = (int)total_staged;
-    HIP_TRY(hipMemcpyAsync(slot->dev, slot->host, args_bytes + 2 * begin_bytes, hipMemcpyHostToDevice, s));
+    tbegin[n] = (int)total_tiles;
+    HIP_TRY(hipMemcpyAsync(slot->dev, slot->host, args_bytes + 3 * begin_bytes, hipMemcpyHostToDevice, s));
     const RadianceArgs* args_dev = (const RadianceArgs*)slot->dev;
+    const int* tbegin_dev = (const int*)((char*)slot->dev + args_bytes + 2 * begin_bytes);
+    if (commit) {
+        HIP_TRY(launch_claim_batch(args_dev, (const int*)((char*)slot->dev + args_bytes + begin_bytes), n,
+                                   (int)total_staged, s));
+        HIP_TRY(launch_compact_batch(args_dev, tbegin_dev, n, (int)total_tiles, s));
+    }
     HIP_TRY(launch_radiance_batch(args_dev, (const int*)((char*)slot->dev + args_bytes), n, (int)total, block, K,
                                   format, served, use_frame ? L->frame_dev : nullptr, s));
     // Subset updates land in their current fields in place.  A trial launch
@@ -225,6 +241,8 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
     if (commit) {
         HIP_TRY(launch_scatter_batch(args_dev, (const int*)((char*)slot->dev + args_bytes + begin_bytes), n,
                                      (int)total_staged, (int)worlds[0]->texel_bytes, s));
+        HIP_TRY(launch_compact_scatter_batch(args_dev, tbegin_dev, n, (int)total_tiles,
+                                             (int)worlds[0]->texel_bytes, s));
         for (int i = 0; i < n; ++i)
             if (!args[i].staged) worlds[i]->cur ^= 1;  // full updates wrote the other field
     }

@@ -50,6 +50,9 @@ bool lumx::stream_capturing(hipStream_t s) {
 }
 
 static const size_t kLdsBudget = 60 * 1024;  // dynamic LDS per tiled workgroup
+// A single-world subset update compacts automatically above this field size
+// (one XCD's L2 holds a smaller field whatever the probe order).
+static const size_t kCompactFieldBytes = size_t(4) << 20;
 
 static lumx_radiance_tuning default_tuning() {
     lumx_radiance_tuning t{};
@@ -288,7 +291,7 @@ void* lumx_context_stream(lumx_context_t c) { return c ? (void*)c->stream : null
 // only changes fields, so a server copies that range out once).
 struct WorldLayout {
     size_t blocks, copy_src, mats, palette, field0, field1, empty_bits, pal_bits, batch_counter, sel_bits,
-        sel_tiles, sel_claim, upd_claim, origin, blockify_state, total;
+        sel_tiles, sel_claim, upd_claim, upd_tiles, origin, blockify_state, total;
 };
 
 static WorldLayout world_layout(const lumx_world* w) {
@@ -314,6 +317,7 @@ static WorldLayout world_layout(const lumx_world* w) {
     L.sel_tiles = take(lumx::select_tiles(w->n_blocks) * sizeof(int));
     L.sel_claim = take(nb * sizeof(int));
     L.upd_claim = take(nb * sizeof(int));
+    L.upd_tiles = take((lumx::select_tiles(w->n_blocks) + 1) * sizeof(int));
     L.origin = take(nb * sizeof(int16_t));
     L.blockify_state = take(4 * sizeof(int));
     L.total = (off + 4095) & ~(size_t)4095;
@@ -335,6 +339,7 @@ static void world_carve(lumx_world* w, char* base) {
     w->sel_tiles = (int*)(base + L.sel_tiles);
     w->sel_claim = (int*)(base + L.sel_claim);
     w->upd_claim = (int*)(base + L.upd_claim);
+    w->upd_tiles = (int*)(base + L.upd_tiles);
     w->origin_dev = (int16_t*)(base + L.origin);
     w->blockify_state = (int*)(base + L.blockify_state);
 }
@@ -1376,6 +1381,7 @@ static int check_tuning(const lumx_world* w, const lumx_radiance_tuning& t, int 
     if (t.variant < 0 || t.variant > 3) return fail(LUMX_EINVAL, "tuning variant %d invalid", t.variant);
     if (t.rays_per_lane != 0 && t.rays_per_lane != 1 && t.rays_per_lane != 2 && t.rays_per_lane != 4)
         return fail(LUMX_EINVAL, "tuning rays_per_lane %d is not 1, 2 or 4", t.rays_per_lane);
+    if (t.compact < -1 || t.compact > 1) return fail(LUMX_EINVAL, "tuning compact %d is not -1, 0 or 1", t.compact);
     if (t.variant == 3 && (t.probes_per_batch < 1 || t.probes_per_batch > 4))
         return fail(LUMX_EINVAL, "tuning probes_per_batch %d invalid for the refill variant (1..4)",
                     t.probes_per_batch);
@@ -1424,6 +1430,13 @@ int lumx::radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx
     a.div_xy = make_divisor((uint32_t)w->desc.size[0] * (uint32_t)w->desc.size[1]);
     return LUMX_OK;
 }
+
+void lumx::compact_args(const lumx_world* w, RadianceArgs* a) {
+    a->compact = 1;
+    a->list_stride = (int)w->texel_bytes;
+    a->tiles = w->upd_tiles;
+    a->n_tiles = (int)select_tiles(w->n_blocks);
+}
 }  // extern "C++"
 
 // Launch one radiance update from field[cur] into field[cur ^ 1] with tuning t.
@@ -1442,18 +1455,33 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
     a.n_requests_dev = count_dev;
     a.ray_keys = keys;
     a.ray_steps = steps;
-    // The scatter that applies a subset update consumes (and resets) the claims.
+    // A subset update claims its probes right before it runs (after every
+    // check that can fail); the scatter that applies it consumes (and resets)
+    // the claims.
     if (a.staged && apply) a.claim = w->upd_claim;
     if (tune.variant == 3 && !traced) {
         if (p->rays != 64) return fail(LUMX_EINVAL, "the refill variant takes 64 rays per probe");
+        HIP_TRY(launch_claim(a, s));
         HIP_TRY(launch_radiance_refill(a, w->desc.radiance_format, tune.probes_per_batch, s));
         if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
         return LUMX_OK;
     }
     if (tune.variant == 0 || tune.variant == 2 || tune.variant == 3) {
         // (a traced launch of the refill variant traces with the lane kernel: the same results)
+        // A large subset update runs over its claimed probes in texel order
+        // (compacted): the same results, better locality.
+        // One world whose field fits a cache gains nothing from texel order,
+        // so automatic compaction waits for larger fields.
+        const bool compact = a.staged && apply && !traced && tune.compact >= 0 &&
+                             (tune.compact > 0 || (compact_pays(a.n_probes, w->n_blocks) &&
+                                                   (size_t)w->n_blocks * w->texel_bytes > kCompactFieldBytes));
+        if (compact) compact_args(w, &a);
+        HIP_TRY(launch_claim(a, s));
+        if (compact) HIP_TRY(launch_compact(a, s));
         HIP_TRY(launch_radiance(a, w->desc.radiance_format, traced, tune.variant == 2, tune.rays_per_lane, s));
-        if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
+        if (a.staged && apply)
+            HIP_TRY(compact ? launch_compact_scatter(a, (int)w->texel_bytes, s)
+                            : launch_scatter(a, (int)w->texel_bytes, s));
         return LUMX_OK;
     }
     RadianceTune t;
@@ -1474,6 +1502,7 @@ static int radiance_launch(lumx_world* w, const int32_t* req, int n_req, const l
         a.pb_words = used_slots * 128;
     }
     if (base > kLdsBudget) return fail(LUMX_EINVAL, "tiled radiance batch does not fit LDS (%zu B)", base);
+    HIP_TRY(launch_claim(a, s));
     HIP_TRY(launch_radiance_tiled(a, w->desc.radiance_format, traced, t, w->ctx->n_cu, s));
     if (a.staged && apply) HIP_TRY(launch_scatter(a, (int)w->texel_bytes, s));
     return LUMX_OK;

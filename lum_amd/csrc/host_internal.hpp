@@ -106,6 +106,8 @@ struct lumx_world {
                                             // device blockify (idle between calls)
     int* upd_claim = nullptr;               // per probe: last request of a subset update that
                                             // named it (-1 between updates)
+    int* upd_tiles = nullptr;               // compacted subset update: per tile of 2048 probes
+                                            // the count, then the offset; then the total
     // Device blockify.
     int16_t* origin_dev = nullptr;          // device copy of the origin world
     bool origin_dirty = true;               // origin changed since its last upload
@@ -175,6 +177,9 @@ int radiance_check(const lumx_world* w, const int32_t* req, int n_req, const lum
 int radiance_params_check(const lumx_radiance_params* p);
 // Kernel arguments of one world's radiance update (field[cur] -> field[cur ^ 1]).
 int radiance_args(lumx_world* w, const int32_t* req, int n_req, const lumx_radiance_params* p, RadianceArgs* a);
+// Marks a staged update's arguments as compacted (claims of the world's
+// subset update, tiles in the world's update scratch).
+void compact_args(const lumx_world* w, RadianceArgs* a);
 // Kernel arguments of one glossy pass over a G-buffer.
 void glossy_args(lumx_world* w, const uint8_t* matnorm, const float* depth, int width, int height,
                  const lumx_camera* cam, float max_dist, int max_steps, float* out, int32_t* keys, GlossyArgs* a);

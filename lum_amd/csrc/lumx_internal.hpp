@@ -23,8 +23,9 @@ struct RadianceArgs {
     int staged;             // 1: probe i's new texel goes to field_out[i] (a subset update,
                             // applied in place by the scatter kernel), 0: to its own texel
     int* claim;             // staged updates followed by a scatter: per texel, the highest
-                            // request index that updated it (-1 between updates), so the
-                            // last of duplicate requests is the one applied; nullptr = none
+                            // request index naming it (set by the claim pass before the
+                            // update, -1 between updates), so the last of duplicate
+                            // requests is the one applied; nullptr = none
     int time_seed;
     int size;
     int shift;
@@ -41,6 +42,14 @@ struct RadianceArgs {
     int probes_per_batch;        // P: probes per batch (power of two)
     int n_batches;
     unsigned int* batch_counter; // 2 words: next batch, finished workgroups (self-resetting)
+    // Compacted subset update (lane kernels): probe j of the launch is the
+    // j-th claimed texel in x-fastest order.  Its entry int2 {texel, request
+    // index} sits at byte j * list_stride of field_out, the slot the probe's
+    // own new texel later overwrites; the request index keys its rays.
+    int compact;
+    int list_stride;             // texel bytes
+    int* tiles;                  // per tile of 2048 texels: count, then offset; [n_tiles] = entries
+    int n_tiles;
 };
 
 // Launch shape of the tiled radiance kernel (chosen by the autotuner).
@@ -204,6 +213,23 @@ hipError_t launch_shift(const ShiftArgs& a, int texel_bytes, hipStream_t stream)
 int rays_per_lane_for(int rays, int want);
 hipError_t launch_radiance(const RadianceArgs& a, int store, bool traced, bool occ, int rays_per_lane,
                            hipStream_t stream);
+// Claims of a staged subset update (a.claim set), launched before it.
+hipError_t launch_claim(const RadianceArgs& a, hipStream_t stream);
+hipError_t launch_claim_batch(const RadianceArgs* args_dev, const int* pbegin_dev, int n_worlds, int total_probes,
+                              hipStream_t stream);
+// Compaction of a claimed subset update into x-fastest texel order, and its
+// scatter (a.compact set): tile counts, per-world scan, entries; after the
+// update, field_in[texel of j] = field_out[j] and the claims reset.  The
+// batched forms take per-world tile ranges tbegin[w] .. tbegin[w + 1] - 1.
+hipError_t launch_compact(const RadianceArgs& a, hipStream_t stream);
+hipError_t launch_compact_scatter(const RadianceArgs& a, int texel_bytes, hipStream_t stream);
+hipError_t launch_compact_batch(const RadianceArgs* args_dev, const int* tbegin_dev, int n_worlds, int total_tiles,
+                                hipStream_t stream);
+hipError_t launch_compact_scatter_batch(const RadianceArgs* args_dev, const int* tbegin_dev, int n_worlds,
+                                        int total_tiles, int texel_bytes, hipStream_t stream);
+// Whether a staged update of n_probes probes on a world of n_blocks is
+// compacted (results are the same either way).
+bool compact_pays(long long n_probes, long long n_blocks);
 // Applies a staged subset update: field_in[probe of i] = field_out[i].
 hipError_t launch_scatter(const RadianceArgs& a, int texel_bytes, hipStream_t stream);
 hipError_t launch_scatter_batch(const RadianceArgs* args_dev, const int* pbegin_dev, int n_worlds, int total_probes,

@@ -59,9 +59,19 @@ __device__ __forceinline__ int dispatched_probes(const RadianceArgs& a) {
     return min(a.n_probes, gld(a.n_requests_dev, 0) / a.size);
 }
 
-// Probe wg's block coordinates (from the request list or the x-fastest
-// index); false when outside the world or not dispatched.
-__device__ __forceinline__ bool probe_coords(const RadianceArgs& a, int wg, int* px, int* py, int* pz) {
+// Probe wg's block coordinates (from the request list, the x-fastest index
+// or the compacted list) and the request index that keys its rays (wg, or
+// the entry's for a compacted update); false when outside the world or not
+// dispatched.
+__device__ __forceinline__ bool probe_coords(const RadianceArgs& a, int wg, int* px, int* py, int* pz, int* key) {
+    *key = wg;
+    if (a.compact) {
+        if (wg >= min(a.n_probes, gld((const int*)a.tiles, (size_t)a.n_tiles))) return false;
+        const int2 e = gld((const int2*)((const char*)a.field_out + (size_t)wg * (size_t)a.list_stride), 0);
+        probe_xyz(a, (uint32_t)e.x, px, py, pz);
+        *key = e.y;
+        return true;
+    }
     if (wg >= dispatched_probes(a)) return false;
     if (a.requests) {
         const int4 q = gld(a.requests, (size_t)(wg * a.size + a.shift));
@@ -75,13 +85,13 @@ __device__ __forceinline__ bool probe_coords(const RadianceArgs& a, int wg, int*
     return (unsigned)*px < (unsigned)w.wx && (unsigned)*py < (unsigned)w.wy && (unsigned)*pz < (unsigned)w.wz;
 }
 
-// Ray r of probe wg at block (px, py, pz) (radiance.comp:370-381): origin
-// and direction.
-__device__ __forceinline__ void ray_setup(const RadianceArgs& a, int wg, int r, int px, int py, int pz, f3* pos_out,
-                                          f3* dir_out) {
+// Ray r of the probe of request index `key` at block (px, py, pz)
+// (radiance.comp:370-381): origin and direction.
+__device__ __forceinline__ void ray_setup(const RadianceArgs& a, int key, int r, int px, int py, int pz,
+                                          f3* pos_out, f3* dir_out) {
     const int R = a.rays;
     const uint32_t t = (uint32_t)a.time_seed;
-    const uint32_t gidx = (uint32_t)wg * (uint32_t)R + (uint32_t)r;
+    const uint32_t gidx = (uint32_t)key * (uint32_t)R + (uint32_t)r;
     f3 r0, r1;
     hash33(gidx + t, t, t, &r0);
     hash33(gidx + t + 1u, t, t, &r1);
@@ -118,13 +128,15 @@ __device__ __forceinline__ f3 shade_ray(const RadianceArgs& a, const RayHit& h, 
     return light;
 }
 
-// Light gathered by ray r of probe wg at block (px, py, pz).
+// Light gathered by ray r of the probe of request index `key` at block
+// (px, py, pz).  Traced launches (never compacted: key is the probe's index
+// in the launch) record the ray at key * R + r.
 template <int STORE, bool TRACED, bool OCC>
-__device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int r, int px, int py, int pz) {
+__device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int key, int r, int px, int py, int pz) {
     const WorldView& w = a.world;
     const int R = a.rays;
     f3 pos, dir;
-    ray_setup(a, wg, r, px, py, pz, &pos, &dir);
+    ray_setup(a, key, r, px, py, pz, &pos, &dir);
 
     RayHit h;
     if (OCC) {
@@ -144,7 +156,7 @@ __device__ __forceinline__ f3 probe_ray_light(const RadianceArgs& a, int wg, int
     }
     const f3 light = shade_ray<STORE>(a, h, pos, dir);
     if (TRACED) {
-        size_t ri = (size_t)wg * R + r;
+        size_t ri = (size_t)key * R + r;
         if (a.ray_keys) gst(a.ray_keys, ri, h.hit ? h.key : 0);
         if (a.ray_steps) {
             gst(a.ray_steps, 2 * ri + 0, h.coarse);
@@ -200,8 +212,6 @@ __device__ __forceinline__ void store_probe(const RadianceArgs& a, int wg, int p
     const float al = a.alpha, bl = 1.0f - a.alpha;
     f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
     Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
-    // Duplicate requests of one probe: the scatter applies the last one.
-    if (a.claim) atomicMax(a.claim + ti, wg);
 }
 
 // K rays per lane.  tid is the thread's index within this world's launch
@@ -227,13 +237,13 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
     // With L >= 64 a wave holds lanes of one probe: keep the probe on the scalar unit.
     if (L >= 64) wg = __builtin_amdgcn_readfirstlane(wg);
     const int m = tid - wg * L;
-    int px = 0, py = 0, pz = 0;
-    const bool valid = probe_coords(a, wg, &px, &py, &pz);
+    int px = 0, py = 0, pz = 0, key = wg;
+    const bool valid = probe_coords(a, wg, &px, &py, &pz, &key);
     const f3 zero = {0.0f, 0.0f, 0.0f};
     const int C = R < 64 ? R : 64;
     f3 total;
     if (K == 1) {
-        f3 light = valid ? probe_ray_light<STORE, TRACED, OCC>(a, wg, m, px, py, pz) : zero;
+        f3 light = valid ? probe_ray_light<STORE, TRACED, OCC>(a, key, m, px, py, pz) : zero;
         total = butterfly(light, C);
         if (R > 64) {
             const int wave = threadIdx.x >> 6;
@@ -261,7 +271,7 @@ __device__ __forceinline__ void radiance_lane_body(const RadianceArgs& a, const 
             const int jc = j / Kw, jj = j - jc * Kw;
             const int o = Kw == 4 ? (((jj & 1) << 1) | (jj >> 1)) : jj;
             const int r = (g * Kc + jc) * C + p + o * Lc;
-            const f3 v = valid ? probe_ray_light<STORE, TRACED, OCC>(a, wg, r, px, py, pz) : zero;
+            const f3 v = valid ? probe_ray_light<STORE, TRACED, OCC>(a, key, r, px, py, pz) : zero;
             if (Kw == 1) {
                 cur = v;
             } else if (Kw == 2) {
@@ -332,11 +342,11 @@ __global__ void __launch_bounds__(256) radiance_refill_kernel(RadianceArgs a, in
     const int wg0 = (blockIdx.x * 4 + wave) * P;
     const WorldView& w = a.world;
     for (int j = 0; j < P; ++j) {
-        int px = 0, py = 0, pz = 0;
+        int px = 0, py = 0, pz = 0, key = 0;
         f3 pos = {-64.0f, -64.0f, -64.0f}, dir = {0.57735027f, 0.57735027f, 0.57735027f};
         // A probe not dispatched or outside the world marches from outside it
         // and leaves at its first sample; its light is not used.
-        if (probe_coords(a, wg0 + j, &px, &py, &pz)) ray_setup(a, wg0 + j, lane, px, py, pz, &pos, &dir);
+        if (probe_coords(a, wg0 + j, &px, &py, &pz, &key)) ray_setup(a, key, lane, px, py, pz, &pos, &dir);
         const MarchRay m = march_ray(pos, dir);
         float* r = rs + (j * 64 + lane) * kRefillFloats;
         r[0] = m.o.x; r[1] = m.o.y; r[2] = m.o.z;
@@ -389,8 +399,8 @@ __global__ void __launch_bounds__(256) radiance_refill_kernel(RadianceArgs a, in
     __syncthreads();
     const f3 zero = {0.0f, 0.0f, 0.0f};
     for (int j = 0; j < P; ++j) {
-        int px = 0, py = 0, pz = 0;
-        const bool valid = probe_coords(a, wg0 + j, &px, &py, &pz);
+        int px = 0, py = 0, pz = 0, key = 0;
+        const bool valid = probe_coords(a, wg0 + j, &px, &py, &pz, &key);
         const int q = j * 64 + lane;
         const MarchRay hm = load_ray(q);
         const float fr = rs[q * kRefillFloats + 9];
@@ -529,16 +539,10 @@ static hipError_t launch_lane(const RadianceArgs& a, int store, bool traced, int
     }
 }
 
-// Scatter of a staged subset update: probe i of the launch (request
-// i * size + shift) gets field_out[i] in field_in, in place.  Probes outside
-// the world were not updated and are skipped, as in the update itself.
-// Where several requests name one probe, the last of them (the highest i,
-// recorded in claim[] by the update) is applied and the claim word is reset
-// for the next update; the others leave the texel alone.
-template <int BYTES>
-__device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
-    typedef typename std::conditional<BYTES == 16, uint4, uint2>::type T;
-    if (i >= dispatched_probes(a)) return;
+// Texel of probe i of a staged subset update (request i * size + shift);
+// false when i is not dispatched or the probe lies outside the world.
+__device__ __forceinline__ bool staged_texel(const RadianceArgs& a, int i, unsigned* ti) {
+    if (i >= dispatched_probes(a)) return false;
     const WorldView& w = a.world;
     int px, py, pz;
     if (a.requests) {
@@ -550,8 +554,32 @@ __device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
         const int p = i * a.size + a.shift;
         probe_xyz(a, (uint32_t)p, &px, &py, &pz);
     }
-    if ((unsigned)px >= (unsigned)w.wx || (unsigned)py >= (unsigned)w.wy || (unsigned)pz >= (unsigned)w.wz) return;
-    const unsigned ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
+    if ((unsigned)px >= (unsigned)w.wx || (unsigned)py >= (unsigned)w.wy || (unsigned)pz >= (unsigned)w.wz) return false;
+    *ti = (unsigned)px + (unsigned)w.wx * ((unsigned)py + (unsigned)w.wy * (unsigned)pz);
+    return true;
+}
+
+// Claims of a staged subset update, before it runs: claim[texel] = the
+// highest i among the probes naming it (duplicate requests of one probe:
+// the last one is applied).  A pass of its own keeps the atomics out of the
+// march kernels, whose waves would otherwise hold their slots until the
+// atomic returns.
+__device__ __forceinline__ void claim_probe(const RadianceArgs& a, int i) {
+    unsigned ti;
+    if (staged_texel(a, i, &ti)) atomicMax(a.claim + ti, i);
+}
+
+// Scatter of a staged subset update: probe i of the launch (request
+// i * size + shift) gets field_out[i] in field_in, in place.  Probes outside
+// the world were not updated and are skipped, as in the update itself.
+// Where several requests name one probe, the last of them (the highest i,
+// recorded in claim[] before the update) is applied and the claim word is
+// reset for the next update; the others leave the texel alone.
+template <int BYTES>
+__device__ __forceinline__ void scatter_probe(const RadianceArgs& a, int i) {
+    typedef typename std::conditional<BYTES == 16, uint4, uint2>::type T;
+    unsigned ti;
+    if (!staged_texel(a, i, &ti)) return;
     if (a.claim) {
         if (a.claim[ti] != i) return;
         a.claim[ti] = -1;
@@ -575,8 +603,258 @@ __global__ void __launch_bounds__(256) scatter_batch_kernel(const RadianceArgs* 
         else hi = mid - 1;
     }
     const RadianceArgs& a = args[lo];
-    if (!a.staged) return;
+    if (!a.staged || a.compact) return;
     scatter_probe<BYTES>(a, t - pbegin[lo]);
+}
+
+__global__ void __launch_bounds__(256) claim_kernel(RadianceArgs a) {
+    claim_probe(a, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// Many worlds: world w owns probes [pbegin[w], pbegin[w + 1]).
+__global__ void __launch_bounds__(256) claim_batch_kernel(const RadianceArgs* args, const int* pbegin, int n) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pbegin[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const RadianceArgs& a = args[lo];
+    if (!a.staged || !a.claim) return;
+    claim_probe(a, t - pbegin[lo]);
+}
+
+hipError_t launch_claim(const RadianceArgs& a, hipStream_t stream) {
+    if (a.n_probes <= 0 || !a.claim) return hipSuccess;
+    hipLaunchKernelGGL(claim_kernel, dim3((a.n_probes + 255) / 256), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_claim_batch(const RadianceArgs* args_dev, const int* pbegin_dev, int n_worlds, int total_probes,
+                              hipStream_t stream) {
+    if (n_worlds <= 0 || total_probes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(claim_batch_kernel, dim3((total_probes + 255) / 256), dim3(256), 0, stream, args_dev,
+                       pbegin_dev, n_worlds);
+    return hipGetLastError();
+}
+
+// ---- compacted subset updates ----
+// The claimed texels of a world (claim >= 0), in x-fastest order, are the
+// probes of the update: tile t holds texels 2048 t .. 2048 t + 2047, 8 rounds
+// of 256 threads; the tile's claimed texels are ranked by round, wave and
+// lane, which is texel order.
+constexpr int kCompactTile = 2048;
+constexpr int kCompactThreads = 256;
+
+bool compact_pays(long long n_probes, long long n_blocks) {
+    // Three passes over the world's claims (about 12 B per texel) against
+    // the update's own work: worth it from a sixteenth of the world and a
+    // few thousand probes up.
+    return n_probes >= 2048 && n_probes * 16 >= n_blocks;
+}
+
+__device__ __forceinline__ int world_texels(const RadianceArgs& a) {
+    return a.world.wx * a.world.wy * a.world.wz;
+}
+
+// Ballot words of tile t's claimed texels (8 rounds x 4 waves, word
+// 4 * round + wave) in LDS; returns the tile's count to thread 0.
+__device__ __forceinline__ int compact_ballots(const RadianceArgs& a, int t, unsigned long long* s_words) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = world_texels(a);
+    int count = 0;
+    for (int k = 0; k < kCompactTile / kCompactThreads; ++k) {
+        const int p = t * kCompactTile + k * kCompactThreads + (int)threadIdx.x;
+        const bool flag = p < n && gld((const int*)a.claim, (size_t)p) >= 0;
+        const unsigned long long m = __ballot(flag);
+        if (lane == 0) s_words[k * (kCompactThreads / 64) + wave] = m;
+        count += __popcll(m);
+    }
+    (void)count;
+    __syncthreads();
+    int total = 0;
+    if (threadIdx.x == 0)
+        for (int c = 0; c < kCompactTile / 64; ++c) total += __popcll(s_words[c]);
+    return total;
+}
+
+// Rank of thread's texel of round k within tile (claimed texels before it).
+__device__ __forceinline__ int compact_rank(const unsigned long long* s_words, const int* s_pref, int k) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = k * (kCompactThreads / 64) + wave;
+    return s_pref[c] + __popcll(s_words[c] & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void compact_prefix(const unsigned long long* s_words, int* s_pref) {
+    if (threadIdx.x < kCompactTile / 64) {
+        int s = 0;
+        for (int c = 0; c < (int)threadIdx.x; ++c) s += __popcll(s_words[c]);
+        s_pref[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void compact_count_tile(const RadianceArgs& a, int t) {
+    __shared__ unsigned long long s_words[kCompactTile / 64];
+    const int total = compact_ballots(a, t, s_words);
+    if (threadIdx.x == 0) a.tiles[t] = total;
+}
+
+// Exclusive scan of one world's tile counts (one workgroup of 1024): the
+// tile offsets, and the total after them.
+__device__ __forceinline__ void compact_scan_world(const RadianceArgs& a) {
+    __shared__ int s_wave[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+    int running = 0;
+    for (int base = 0; base < a.n_tiles; base += blockDim.x) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < a.n_tiles ? a.tiles[i] : 0;
+        int x = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            int w = lane < n_waves ? s_wave[lane] : 0;
+            for (int d = 1; d < 16; d <<= 1) {
+                const int y = __shfl_up(w, d);
+                if (lane >= d) w += y;
+            }
+            if (lane < n_waves) s_wave[lane] = w;
+        }
+        __syncthreads();
+        const int before = wave > 0 ? s_wave[wave - 1] : 0;
+        if (i < a.n_tiles) a.tiles[i] = running + before + x - v;
+        running += s_wave[n_waves - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.tiles[a.n_tiles] = running;
+}
+
+// Entries {texel, request index} of tile t's claimed texels.
+__device__ __forceinline__ void compact_list_tile(const RadianceArgs& a, int t) {
+    __shared__ unsigned long long s_words[kCompactTile / 64];
+    __shared__ int s_pref[kCompactTile / 64];
+    compact_ballots(a, t, s_words);
+    compact_prefix(s_words, s_pref);
+    const int offset = a.tiles[t];
+    const int n = world_texels(a);
+    for (int k = 0; k < kCompactTile / kCompactThreads; ++k) {
+        const int p = t * kCompactTile + k * kCompactThreads + (int)threadIdx.x;
+        if (p >= n) break;
+        const int c = gld((const int*)a.claim, (size_t)p);
+        if (c < 0) continue;
+        const int j = offset + compact_rank(s_words, s_pref, k);
+        gst((int2*)((char*)a.field_out + (size_t)j * (size_t)a.list_stride), 0, make_int2(p, c));
+    }
+}
+
+// After the update: field_in[texel of entry j] = field_out[j], claims reset.
+template <int BYTES>
+__device__ __forceinline__ void compact_scatter_tile(const RadianceArgs& a, int t) {
+    typedef typename std::conditional<BYTES == 16, uint4, uint2>::type T;
+    __shared__ unsigned long long s_words[kCompactTile / 64];
+    __shared__ int s_pref[kCompactTile / 64];
+    compact_ballots(a, t, s_words);
+    compact_prefix(s_words, s_pref);
+    const int offset = a.tiles[t];
+    const int n = world_texels(a);
+    for (int k = 0; k < kCompactTile / kCompactThreads; ++k) {
+        const int p = t * kCompactTile + k * kCompactThreads + (int)threadIdx.x;
+        if (p >= n) break;
+        if (gld((const int*)a.claim, (size_t)p) < 0) continue;
+        const int j = offset + compact_rank(s_words, s_pref, k);
+        gst((T*)a.field_in, (size_t)p, gld((const T*)a.field_out, (size_t)j));
+        gst(a.claim, (size_t)p, -1);
+    }
+}
+
+__global__ void __launch_bounds__(kCompactThreads) compact_count_kernel(RadianceArgs a) {
+    compact_count_tile(a, (int)blockIdx.x);
+}
+__global__ void __launch_bounds__(1024) compact_scan_kernel(RadianceArgs a) { compact_scan_world(a); }
+__global__ void __launch_bounds__(kCompactThreads) compact_list_kernel(RadianceArgs a) {
+    compact_list_tile(a, (int)blockIdx.x);
+}
+template <int BYTES>
+__global__ void __launch_bounds__(kCompactThreads) compact_scatter_kernel(RadianceArgs a) {
+    compact_scatter_tile<BYTES>(a, (int)blockIdx.x);
+}
+
+// Many worlds: world w owns tiles tbegin[w] .. tbegin[w + 1] - 1 (none when
+// not compacted); a workgroup finds its world by a scalar search.
+__device__ __forceinline__ int tile_world(const int* tbegin, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tbegin[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ void __launch_bounds__(kCompactThreads) compact_count_batch_kernel(const RadianceArgs* args,
+                                                                              const int* tbegin, int n) {
+    const int w = tile_world(tbegin, n, (int)blockIdx.x);
+    compact_count_tile(args[w], (int)blockIdx.x - tbegin[w]);
+}
+__global__ void __launch_bounds__(1024) compact_scan_batch_kernel(const RadianceArgs* args) {
+    const RadianceArgs& a = args[blockIdx.x];
+    if (a.compact) compact_scan_world(a);
+}
+__global__ void __launch_bounds__(kCompactThreads) compact_list_batch_kernel(const RadianceArgs* args,
+                                                                             const int* tbegin, int n) {
+    const int w = tile_world(tbegin, n, (int)blockIdx.x);
+    compact_list_tile(args[w], (int)blockIdx.x - tbegin[w]);
+}
+template <int BYTES>
+__global__ void __launch_bounds__(kCompactThreads) compact_scatter_batch_kernel(const RadianceArgs* args,
+                                                                                const int* tbegin, int n) {
+    const int w = tile_world(tbegin, n, (int)blockIdx.x);
+    compact_scatter_tile<BYTES>(args[w], (int)blockIdx.x - tbegin[w]);
+}
+
+hipError_t launch_compact(const RadianceArgs& a, hipStream_t stream) {
+    if (!a.compact || a.n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_count_kernel, dim3(a.n_tiles), dim3(kCompactThreads), 0, stream, a);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, stream, a);
+    hipLaunchKernelGGL(compact_list_kernel, dim3(a.n_tiles), dim3(kCompactThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_scatter(const RadianceArgs& a, int texel_bytes, hipStream_t stream) {
+    if (!a.compact || a.n_tiles <= 0) return hipSuccess;
+    if (texel_bytes == 16)
+        hipLaunchKernelGGL(compact_scatter_kernel<16>, dim3(a.n_tiles), dim3(kCompactThreads), 0, stream, a);
+    else
+        hipLaunchKernelGGL(compact_scatter_kernel<8>, dim3(a.n_tiles), dim3(kCompactThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_batch(const RadianceArgs* args_dev, const int* tbegin_dev, int n_worlds, int total_tiles,
+                                hipStream_t stream) {
+    if (n_worlds <= 0 || total_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_count_batch_kernel, dim3(total_tiles), dim3(kCompactThreads), 0, stream, args_dev,
+                       tbegin_dev, n_worlds);
+    hipLaunchKernelGGL(compact_scan_batch_kernel, dim3(n_worlds), dim3(1024), 0, stream, args_dev);
+    hipLaunchKernelGGL(compact_list_batch_kernel, dim3(total_tiles), dim3(kCompactThreads), 0, stream, args_dev,
+                       tbegin_dev, n_worlds);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_scatter_batch(const RadianceArgs* args_dev, const int* tbegin_dev, int n_worlds,
+                                        int total_tiles, int texel_bytes, hipStream_t stream) {
+    if (n_worlds <= 0 || total_tiles <= 0) return hipSuccess;
+    if (texel_bytes == 16)
+        hipLaunchKernelGGL(compact_scatter_batch_kernel<16>, dim3(total_tiles), dim3(kCompactThreads), 0, stream,
+                           args_dev, tbegin_dev, n_worlds);
+    else
+        hipLaunchKernelGGL(compact_scatter_batch_kernel<8>, dim3(total_tiles), dim3(kCompactThreads), 0, stream,
+                           args_dev, tbegin_dev, n_worlds);
+    return hipGetLastError();
 }
 
 hipError_t launch_scatter(const RadianceArgs& a, int texel_bytes, hipStream_t stream) {
@@ -780,7 +1058,6 @@ __global__ void __launch_bounds__(1024) radiance_tiled_kernel(RadianceArgs a) {
             const float al = a.alpha, bl = 1.0f - a.alpha;
             f3 nv = {old.x * bl + mean.x * al, old.y * bl + mean.y * al, old.z * bl + mean.z * al};
             Texel<STORE>::store(a.field_out, a.staged ? (unsigned)wg : ti, unorm_nan_to_zero(nv));
-            if (a.claim) atomicMax(a.claim + ti, wg);
         }
     }
     // The last workgroup to finish resets the batch counter for the next launch.

@@ -78,6 +78,45 @@ def test_batch_radiance_equals_single_worlds(gpu_ctx):
             w.close()
 
 
+def test_batch_compacted_subset_updates_equal_single(gpu_ctx):
+    """Subset updates large enough for the batch to compact them (claimed
+    probes in texel order) beside a small one it does not compact: each
+    world equals its own single-world update run without compaction, bit for
+    bit, with repeated and outside entries, over two frames."""
+    import torch
+    specs = [((16, 16, 16), 4096, 1500, 61), ((24, 20, 10), 3000, 2500, 62), ((12, 10, 6), 300, 40, 63)]
+    bw, sw, reqs = [], [], []
+    for size, n, distinct, seed in specs:
+        b, _ = make_world(gpu_ctx, size, seed)
+        s, _ = make_world(gpu_ctx, size, seed)
+        s.set_tuning(variant=0, compact=-1)
+        bw.append(b)
+        sw.append(s)
+        wx, wy, wz = size
+        rng = np.random.default_rng(seed + 7)
+        flat = rng.integers(0, distinct, size=n)
+        q = np.stack([flat % wx, (flat // wx) % wy, flat // (wx * wy), np.zeros(n, int)], 1).astype(np.int32)
+        q[3] = [wx, 0, 0, 0]
+        reqs.append(torch.from_numpy(q).cuda())
+    batch = Batch(gpu_ctx, bw)
+    try:
+        params = [radiance_params(rays=r, time_seed=40 + i, alpha=0.3, max_steps=64)
+                  for i, r in enumerate((64, 128, 32))]
+        for frame in range(2):
+            batch.radiance(params, reqs)
+            for i, s in enumerate(sw):
+                s.radiance_update(reqs[i], params[i])
+            for p in params:
+                p.time_seed += 1
+        torch.cuda.synchronize()
+        for i, (x, y) in enumerate(zip(bw, sw)):
+            assert np.array_equal(x.read_radiance(), y.read_radiance()), i
+    finally:
+        batch.close()
+        for w in bw + sw:
+            w.close()
+
+
 def test_batch_frame_counter_offsets_seed(gpu_ctx):
     import torch
     w, _ = make_world(gpu_ctx, (16, 16, 8), 21)
