@@ -371,7 +371,7 @@ int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_mod
                 const MapArgs& x = args[i][j];
                 a[c] = x;
                 begin[c] = (int)g;
-                g += (long long)x.tiles_x * x.tiles_y * (x.nz / 4);
+                g += map_tiles(x);
                 ++c;
             }
         }

@@ -1025,7 +1025,7 @@ static int plan_map_launches(lumx_world* w, int n, const lumx_model_t* models, c
         a.cls = 7;
         a.tiles_x = (a.nx + 15) / 16;
         a.tiles_y = a.ny / 4;
-        if ((long long)a.tiles_x * a.tiles_y * (a.nz / 4) >= (1ll << 28))
+        if (map_tiles(a) >= (1ll << 28))
             return fail(LUMX_EINVAL, "model %d dispatch too large", i);
         a.div_tiles_x = make_divisor((uint32_t)std::max(a.tiles_x, 1));
         a.div_tiles_y = make_divisor((uint32_t)std::max(a.tiles_y, 1));
@@ -1046,7 +1046,7 @@ static int plan_map_launches(lumx_world* w, int n, const lumx_model_t* models, c
         for (int i = 0; i < n; ++i) {
             if (layer[i] != l) continue;
             const MapArgs& a = args[i];
-            const long long groups = (long long)a.tiles_x * a.tiles_y * (a.nz / 4);
+            const long long groups = map_tiles(a);
             if (groups <= 0) continue;
             if (negative[i]) {
                 // Alone in its layer: one launch per sign class present.
@@ -1085,7 +1085,7 @@ static int launch_models(lumx_world* w, int n, const lumx_model_t* models, const
         for (int k = 0; k <= count; ++k) {
             if (k < count) {
                 const MapArgs& a = group[k];
-                const long long groups = (long long)a.tiles_x * a.tiles_y * (a.nz / 4);
+                const long long groups = map_tiles(a);
                 if ((long long)g.begin[g.n] + groups > (1ll << 31) - 1)
                     return fail(LUMX_EINVAL, "map launch too large");
                 g.m[g.n] = a;

@@ -73,9 +73,16 @@ struct MapArgs {
     int shift_x, shift_y, shift_z;
     float m[12];            // rows 0..2 of the inverse transform: m[r*4 + c]
     int cls;                // sign class of the threads this launch maps (7: all coordinates >= 0)
-    int tiles_x, tiles_y;   // 16 x 4 x 4 thread tiles per axis: (nx + 15) / 16, ny / 4
+    int tiles_x, tiles_y;   // workgroup tiles of 16 x 4 x kMapTileZ voxels: (nx + 15) / 16, ny / 4
     Divisor div_tiles_x, div_tiles_y;
 };
+
+// A map workgroup of 256 threads covers 16 x 4 x kMapTileZ voxels of a
+// model's dispatch box, each thread kMapTileZ / 4 of them along z.
+constexpr int kMapTileZ = 16;
+inline long long map_tiles(const MapArgs& a) {
+    return (long long)a.tiles_x * a.tiles_y * ((a.nz + kMapTileZ - 1) / kMapTileZ);
+}
 
 // Models of one map launch (kernel argument, at most kMapGroup models).
 constexpr int kMapGroup = 16;

@@ -1,7 +1,8 @@
 // map.hip - model->world voxelization and palette slot copies (gfx950).
 //
-// map_group_kernel restates shaders/map.comp:33-68 of the reference: one thread per
-// candidate world voxel of the model's dispatch box, FP32 inverse transform
+// map_group_kernel restates shaders/map.comp:33-68 of the reference: per
+// candidate world voxel of the model's dispatch box (one shader invocation
+// each; here a thread takes four, see map_tile), FP32 inverse transform
 // (unfused, rows evaluated ((m0*x + m1*y) + m2*z) + m3), truncation to model
 // voxel, predicated byte store into the palette slot the blockified world
 // assigns to that block.
@@ -28,10 +29,11 @@
 
 namespace lumx {
 
-// Thread (gx, gy, gz) of the model's dispatch box.
-__device__ __forceinline__ void map_thread(const MapArgs& a, int gx, int gy, int gz) {
+// Voxel (gx, gy, gz) of the model's dispatch box: loads its block id and
+// model voxel; true (with the palette byte offset and value) when it stores.
+__device__ __forceinline__ bool map_voxel(const MapArgs& a, int gx, int gy, int gz, uint32_t* off, uint8_t* val) {
     const int cls = (gx + a.shift_x >= 0 ? 1 : 0) | (gy + a.shift_y >= 0 ? 2 : 0) | (gz + a.shift_z >= 0 ? 4 : 0);
-    if (cls != a.cls) return;
+    if (cls != a.cls) return false;
 
     const float wvx = ((float)gx + 0.5f) + (float)a.shift_x;
     const float wvy = ((float)gy + 0.5f) + (float)a.shift_y;
@@ -56,29 +58,43 @@ __device__ __forceinline__ void map_thread(const MapArgs& a, int gx, int gy, int
     uint8_t v = 0;
     if ((unsigned)imx < (unsigned)a.mx && (unsigned)imy < (unsigned)a.my && (unsigned)imz < (unsigned)a.mz)
         v = a.model[imx + (size_t)a.mx * (imy + (size_t)a.my * imz)];
-    if (target < a.static_count || target >= a.palette_slots || v == 0) return;
-    const uint32_t off = ((uint32_t)target << 12) | (uint32_t)((iwx & 15) | ((iwy & 15) << 4) | ((iwz & 15) << 8));
-    a.palette[off] = v;
+    if (target < a.static_count || target >= a.palette_slots || v == 0) return false;
+    *off = ((uint32_t)target << 12) | (uint32_t)((iwx & 15) | ((iwy & 15) << 4) | ((iwz & 15) << 8));
+    *val = v;
+    return true;
 }
 
 // Several models whose write boxes are disjoint in one launch: model i owns
 // workgroups [begin[i], begin[i + 1]).  Disjoint boxes write disjoint palette
 // bytes, so the result equals mapping them one after another.  A workgroup of
-// 256 threads covers a 16 x 4 x 4 tile of its model's dispatch box (x
-// fastest, so a row of 16 threads stores into one 16-byte palette row); the
-// tile is found by 32-bit divisions of the workgroup index, once per
+// 256 threads covers a 16 x 4 x kMapTileZ tile of its model's dispatch box
+// (x fastest, so a row of 16 threads stores into one 16-byte palette row);
+// each thread takes kMapTileZ / 4 voxels 4 apart in z, issuing all their
+// loads before any store.  Within one sign class no two voxels of a model
+// write one palette byte, so which thread takes which voxel does not matter.
+// The tile is found by 32-bit divisions of the workgroup index, once per
 // workgroup on the scalar unit.
-// Tile t of model a's dispatch box (16 x 4 x 4 threads, this thread's part).
 __device__ __forceinline__ void map_tile(const MapArgs& a, uint32_t t) {
+    constexpr int kPer = kMapTileZ / 4;
     const uint32_t r = fast_div(t, a.div_tiles_x);
     const uint32_t bx = t - r * (uint32_t)a.tiles_x;
     const uint32_t bz = fast_div(r, a.div_tiles_y);
     const uint32_t by = r - bz * (uint32_t)a.tiles_y;
     const int gx = (int)(bx * 16u + (threadIdx.x & 15u));
     const int gy = (int)(by * 4u + ((threadIdx.x >> 4) & 3u));
-    const int gz = (int)(bz * 4u + (threadIdx.x >> 6));
+    const int gz0 = (int)(bz * (uint32_t)kMapTileZ + (threadIdx.x >> 6));
     if (gx >= a.nx) return;
-    map_thread(a, gx, gy, gz);
+    uint32_t off[kPer];
+    uint8_t val[kPer];
+    bool st[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int gz = gz0 + 4 * k;
+        st[k] = gz < a.nz && map_voxel(a, gx, gy, gz, &off[k], &val[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (st[k]) a.palette[off[k]] = val[k];
 }
 
 __global__ void __launch_bounds__(256) map_group_kernel(MapGroup g) {
