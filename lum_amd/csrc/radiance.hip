@@ -701,8 +701,9 @@ __device__ __forceinline__ void compact_count_tile(const RadianceArgs& a, int t)
     if (threadIdx.x == 0) a.tiles[t] = total;
 }
 
-// Exclusive scan of one world's tile counts (one workgroup of 1024): the
-// tile offsets, and the total after them.
+// Exclusive scan of one world's tile counts (one workgroup of 256, small
+// enough to start between the waves of a busy GPU): the tile offsets, and
+// the total after them.
 __device__ __forceinline__ void compact_scan_world(const RadianceArgs& a) {
     __shared__ int s_wave[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
@@ -775,7 +776,7 @@ __device__ __forceinline__ void compact_scatter_tile(const RadianceArgs& a, int 
 __global__ void __launch_bounds__(kCompactThreads) compact_count_kernel(RadianceArgs a) {
     compact_count_tile(a, (int)blockIdx.x);
 }
-__global__ void __launch_bounds__(1024) compact_scan_kernel(RadianceArgs a) { compact_scan_world(a); }
+__global__ void __launch_bounds__(kCompactThreads) compact_scan_kernel(RadianceArgs a) { compact_scan_world(a); }
 __global__ void __launch_bounds__(kCompactThreads) compact_list_kernel(RadianceArgs a) {
     compact_list_tile(a, (int)blockIdx.x);
 }
@@ -801,7 +802,7 @@ __global__ void __launch_bounds__(kCompactThreads) compact_count_batch_kernel(co
     const int w = tile_world(tbegin, n, (int)blockIdx.x);
     compact_count_tile(args[w], (int)blockIdx.x - tbegin[w]);
 }
-__global__ void __launch_bounds__(1024) compact_scan_batch_kernel(const RadianceArgs* args) {
+__global__ void __launch_bounds__(kCompactThreads) compact_scan_batch_kernel(const RadianceArgs* args) {
     const RadianceArgs& a = args[blockIdx.x];
     if (a.compact) compact_scan_world(a);
 }
@@ -820,7 +821,7 @@ __global__ void __launch_bounds__(kCompactThreads) compact_scatter_batch_kernel(
 hipError_t launch_compact(const RadianceArgs& a, hipStream_t stream) {
     if (!a.compact || a.n_tiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(compact_count_kernel, dim3(a.n_tiles), dim3(kCompactThreads), 0, stream, a);
-    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(1024), 0, stream, a);
+    hipLaunchKernelGGL(compact_scan_kernel, dim3(1), dim3(kCompactThreads), 0, stream, a);
     hipLaunchKernelGGL(compact_list_kernel, dim3(a.n_tiles), dim3(kCompactThreads), 0, stream, a);
     return hipGetLastError();
 }
@@ -839,7 +840,7 @@ hipError_t launch_compact_batch(const RadianceArgs* args_dev, const int* tbegin_
     if (n_worlds <= 0 || total_tiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(compact_count_batch_kernel, dim3(total_tiles), dim3(kCompactThreads), 0, stream, args_dev,
                        tbegin_dev, n_worlds);
-    hipLaunchKernelGGL(compact_scan_batch_kernel, dim3(n_worlds), dim3(1024), 0, stream, args_dev);
+    hipLaunchKernelGGL(compact_scan_batch_kernel, dim3(n_worlds), dim3(kCompactThreads), 0, stream, args_dev);
     hipLaunchKernelGGL(compact_list_batch_kernel, dim3(total_tiles), dim3(kCompactThreads), 0, stream, args_dev,
                        tbegin_dev, n_worlds);
     return hipGetLastError();
