@@ -80,7 +80,8 @@ def radiance_bytes(keys, steps, n_probes, texel_bytes, request_bytes=0):
     """Gathered-bytes model of one radiance launch: 3 B (int16 block id +
     u8 voxel) per coarse sample and refinement fetch, 20 B of material plus
     8 trilinear texels per hit, and per probe the old texel read, the new
-    texel written and its request."""
+    texel written and its request.  keys and steps are numpy arrays or
+    device tensors (reduced where they live)."""
     samples = int(steps[:, 0].sum()) + int(steps[:, 1].sum())
     hits = int((keys != 0).sum())
     return 3 * samples + hits * (20 + 8 * texel_bytes) + n_probes * (2 * texel_bytes + request_bytes)
@@ -173,7 +174,7 @@ def run_plain(args):
     # algorithmic bytes of one launch (traced, untimed)
     keys, steps = w.radiance_update(None, params(0), traced=True)
     torch.cuda.synchronize()
-    bytes_per_launch = radiance_bytes(keys.cpu().numpy(), steps.cpu().numpy(), n_probes, 16)
+    bytes_per_launch = radiance_bytes(keys, steps, n_probes, 16)
 
     for i in range(args.warmup):
         w.radiance_update(None, params(1 + i))
@@ -523,7 +524,7 @@ def run_serving(args):
     bytes_per_launch = 0
     for w in worlds:
         keys, steps = w.radiance_update(None, rp[0], traced=True)
-        bytes_per_launch += radiance_bytes(keys.cpu().numpy(), steps.cpu().numpy(), w.n_blocks, 8)
+        bytes_per_launch += radiance_bytes(keys, steps, w.n_blocks, 8)
     for _ in range(2):
         batch.radiance(rp)
     torch.cuda.synchronize()
