@@ -429,15 +429,17 @@ int launcher_glossy(Launcher* L, lumx_world* const* worlds, int n, const uint8_t
     const size_t args_bytes = round_up((size_t)n * record_bytes(), 256);
     GlossyArgs* args = (GlossyArgs*)slot->host;
     int* begin = (int*)((char*)slot->host + args_bytes);
-    long long total = 0;
+    long long total = 0;  // 16 x 16 pixel tiles
     for (int i = 0; i < n; ++i) {
         glossy_args(worlds[i], matnorm[i], depth[i], width[i], height[i], &cams[i], max_dist[i], max_steps[i], out[i],
                     nullptr, &args[i]);
         begin[i] = (int)total;
-        total += round_up((size_t)width[i] * height[i], 256);
-        if (total > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batch too large");
+        total += (long long)((width[i] + 15) / 16) * ((height[i] + 15) / 16);
+        if (total * 256 > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batch too large");
     }
-    HIP_TRY(hipMemcpyAsync(slot->dev, slot->host, args_bytes + (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+    begin[n] = (int)total;
+    HIP_TRY(hipMemcpyAsync(slot->dev, slot->host, args_bytes + ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice,
+                           s));
     HIP_TRY(launch_glossy_batch((const GlossyArgs*)slot->dev, (const int*)((char*)slot->dev + args_bytes), n,
                                 (int)total, format, s));
     return release_slot(slot, s);

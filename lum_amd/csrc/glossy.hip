@@ -68,29 +68,33 @@ __global__ void __launch_bounds__(256) glossy_kernel(GlossyArgs a) {
     glossy_pixel<STORE, TRACED>(a, px, py);
 }
 
-// Many frames in one launch: frame w owns threads [begin[w], begin[w] + w*h),
-// each range starting on a workgroup boundary, pixels row-major.
+// Many frames in one launch: frame w owns workgroups [begin[w], begin[w + 1]),
+// one 16 x 16 pixel tile each, tiles row-major (the single-frame kernel's
+// tiling, so a wave's reflection rays start from a 16 x 4 patch).
 template <int STORE>
 __global__ void __launch_bounds__(256) glossy_batch_kernel(const GlossyArgs* args, const int* begin, int n) {
-    const int b0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x));
+    const int b = (int)blockIdx.x;
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (begin[mid] <= b0) lo = mid;
+        if (begin[mid] <= b) lo = mid;
         else hi = mid - 1;
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
     const GlossyArgs a = args[lo];
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - begin[lo];
-    if (i >= a.width * a.height) return;
-    const int py = i / a.width;
-    glossy_pixel<STORE, false>(a, i - py * a.width, py);
+    const int t = b - begin[lo];
+    const int tiles_x = (a.width + 15) >> 4;
+    const int ty = t / tiles_x;
+    const int px = (t - ty * tiles_x) * 16 + (int)(threadIdx.x & 15);
+    const int py = ty * 16 + (int)(threadIdx.x >> 4);
+    if (px >= a.width || py >= a.height) return;
+    glossy_pixel<STORE, false>(a, px, py);
 }
 
-hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_threads, int store,
+hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_tiles, int store,
                                hipStream_t stream) {
-    if (n <= 0 || total_threads <= 0) return hipSuccess;
-    const int grid = (total_threads + 255) / 256;
+    if (n <= 0 || total_tiles <= 0) return hipSuccess;
+    const int grid = total_tiles;
     switch (store) {
         case STORE_F16:
             hipLaunchKernelGGL(glossy_batch_kernel<STORE_F16>, dim3(grid), dim3(256), 0, stream, args_dev, begin_dev, n);

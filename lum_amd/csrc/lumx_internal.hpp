@@ -287,7 +287,8 @@ hipError_t launch_diffuse(const DiffuseArgs& a, int store, hipStream_t stream);
 hipError_t launch_radiance_batch(const RadianceArgs* args_dev, const int* begin_dev, int n_worlds,
                                  int total_threads, int block, int rays_per_lane, int store, bool served,
                                  const int* frame_dev, hipStream_t stream);
-hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_threads, int store,
+// Frame w owns 16 x 16 pixel tiles begin[w] .. begin[w + 1] - 1 (one workgroup each).
+hipError_t launch_glossy_batch(const GlossyArgs* args_dev, const int* begin_dev, int n, int total_tiles, int store,
                                hipStream_t stream);
 
 }  // namespace lumx

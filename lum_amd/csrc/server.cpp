@@ -156,8 +156,8 @@ static bool resident(const lumx_world* w) { return w->blocks != nullptr; }
 
 // Threads a request adds to its batched launch, bounded above: a radiance
 // request takes n_probes * rays / K threads padded to the workgroup (K >= 1,
-// workgroups of at most 1024), a glossy one w * h padded to 256.  A launch's
-// total must stay below 2^31.
+// workgroups of at most 1024), a glossy one 256 per 16 x 16 pixel tile.  A
+// launch's total must stay below 2^31.
 static const long long kLaunchThreads = (1ll << 31) - 1;
 static long long request_threads(const lumx_request& r, const lumx_world* w) {
     if (r.kind == LUMX_REQ_RADIANCE) {
@@ -165,7 +165,7 @@ static long long request_threads(const lumx_request& r, const lumx_world* w) {
         const long long probes = n / std::max(r.params.size, 1);
         return (probes * r.params.rays + 1023) / 1024 * 1024;
     }
-    return ((long long)r.width * r.height + 255) / 256 * 256;
+    return (long long)((r.width + 15) / 16) * ((r.height + 15) / 16) * 256;
 }
 
 // Make world i resident within the budget (caller holds the lock; `taken`
