@@ -250,11 +250,13 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
 }
 
 // Batched map: per world the host half of lumx_map_models_placed
-// (map_prepare), then for all worlds together one upload of the blockified
-// worlds, copy lists and launch descriptors, one commit launch (blocks,
-// empty-block bits, copy lists, counters), one exec_copies launch, the map
-// launches (group k of every world in launch k: groups of one world stay in
-// order, different worlds write disjoint memory) and one palette-bits launch.
+// (map_prepare), then for all worlds together one upload of the allocated
+// slots' blocks, copy lists and launch descriptors (and of an origin world
+// that changed), one commit launch (the origin into the blocks with its
+// empty-block bits, copy lists, counters), one exec_copies launch that also
+// gives each allocated slot's block its id, the map launches (group k of
+// every world in launch k: groups of one world stay in order, different
+// worlds write disjoint memory) and one palette-bits launch.
 int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_models,
                  const lumx_model_t* const* models, const lumx_placement* const* pl, hipStream_t s) {
     if (n <= 0) return LUMX_OK;
@@ -317,6 +319,7 @@ int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_mod
         const int hi = std::max(w->counter, w->prev_counter);
         for (int k = w->counter; k < hi; ++k) w->copy_src[k] = 0;
         d.blocks = w->blocks;
+        d.origin = w->origin_dev;
         d.copy_src = w->copy_src_dev;
         d.state = w->blockify_state;
         d.empty_bits = w->empty_bits;
@@ -326,10 +329,17 @@ int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_mod
         d.st = st;
         d.hi = std::max(hi, st);
         d.counter = w->counter;
-        d.stage_blocks = off;
-        off += round_up((size_t)w->n_blocks * 2, 256);
+        // The origin world crosses PCIe only when it changed since its last
+        // upload; the blockified world is rebuilt from it on the device.
+        d.stage_origin = -1;
+        if (w->origin_dirty) {
+            d.stage_origin = (long long)off;
+            off += round_up((size_t)w->n_blocks * 2, 256);
+        }
         d.stage_copy = off;
         off += round_up((size_t)(d.hi - st) * 4, 256);
+        d.stage_slots = off;
+        off += round_up((size_t)std::max(w->counter - st, 0) * 4, 256);
         d.group_begin = (int)groups;
         groups += (map_commit_elements(d) + 255) / 256;
         d.slot_begin = (int)slots;
@@ -380,16 +390,21 @@ int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_mod
         total_groups[k] = (int)g;
     }
     for (int i = 0; i < m; ++i) {
-        const lumx_world* w = worlds[i];
-        std::memcpy(h + desc[i].stage_blocks, w->current.data(), (size_t)w->n_blocks * 2);
-        if (desc[i].hi > desc[i].st)
-            std::memcpy(h + desc[i].stage_copy, w->copy_src.data() + desc[i].st,
-                        (size_t)(desc[i].hi - desc[i].st) * 4);
+        lumx_world* w = worlds[i];
+        const MapWorldDesc& d = desc[i];
+        if (d.stage_origin >= 0) {
+            std::memcpy(h + d.stage_origin, w->origin.data(), (size_t)w->n_blocks * 2);
+            // A recorded upload runs only when the graph does: eager maps keep uploading.
+            if (!capturing) w->origin_dirty = false;
+        }
+        if (d.hi > d.st) std::memcpy(h + d.stage_copy, w->copy_src.data() + d.st, (size_t)(d.hi - d.st) * 4);
+        if (d.counter > d.st)
+            std::memcpy(h + d.stage_slots, w->slot_block.data() + d.st, (size_t)(d.counter - d.st) * 4);
     }
     HIP_TRY(hipMemcpyAsync(dv, h, need, hipMemcpyHostToDevice, s));
     const MapWorldDesc* d_dev = (const MapWorldDesc*)dv;
     HIP_TRY(launch_map_commit_batch(d_dev, m, (int)groups, dv, s));
-    HIP_TRY(launch_map_copies_batch(d_dev, m, (int)slots, s));
+    HIP_TRY(launch_map_copies_batch(d_dev, m, (int)slots, dv, s));
     for (size_t k = 0; k < n_launches; ++k)
         HIP_TRY(launch_map_batch((const MapArgs*)(dv + launch_off[k]), (const int*)(dv + launch_begin_off[k]),
                                  launch_count[k], total_groups[k], s));

@@ -527,6 +527,7 @@ int lumx_world_create(lumx_context_t c, const lumx_world_desc* d, lumx_world_t* 
     w->origin.assign(nb, 0);
     w->current.assign(nb, 0);
     w->copy_src.assign(slots, 0);
+    w->slot_block.assign(slots, 0);
     w->static_palette.assign((size_t)d->static_blocks * 4096, 0);
     w->counter = d->static_blocks;
     w->prev_counter = d->static_blocks;
@@ -968,11 +969,13 @@ static int blockify(lumx_world* w, const lumx_placement& p) {
     for (int z = mn[2]; z <= mx[2]; ++z)
         for (int y = mn[1]; y <= mx[1]; ++y)
             for (int x = mn[0]; x <= mx[0]; ++x) {
-                int16_t& b = w->current[x + wx * (y + wy * z)];
+                const int idx = x + wx * (y + wy * z);
+                int16_t& b = w->current[idx];
                 if (b < w->desc.static_blocks) {
                     if (w->counter >= w->desc.palette_slots)
                         return fail(LUMX_EOVERFLOW, "block palette full (%d slots)", w->desc.palette_slots);
                     w->copy_src[w->counter] = b;
+                    w->slot_block[w->counter] = idx;
                     b = (int16_t)w->counter;
                     w->counter++;
                 }

@@ -84,6 +84,7 @@ struct lumx_world {
     std::vector<int16_t> origin;   // CPU origin world (Lum's origin_world)
     std::vector<int16_t> current;  // blockified world of the last map
     std::vector<int32_t> copy_src; // per palette slot: static source block
+    std::vector<int32_t> slot_block; // per palette slot the last host blockify allocated: its block
     int counter = 0;               // palette slots in use (paletteCounter)
     bool block0_empty = true;
     std::vector<uint8_t> static_palette;  // host copy of the static slots

@@ -253,29 +253,37 @@ hipError_t launch_map_group(const MapGroup& g, hipStream_t stream);
 
 // Batched map over many worlds (map.hip): per world the staged result of its
 // host blockify and where it goes.
+// The blockified world is its origin world with block slot_block[k] set to
+// k for each slot k in st .. counter - 1 (what blockify allocated), so only
+// that list crosses PCIe; the origin comes from the device copy, or from the
+// staging (and is written to the device copy) when it changed.
 struct MapWorldDesc {
     int16_t* blocks;          // device: blockified world
+    int16_t* origin;          // device: copy of the origin world
     int32_t* copy_src;        // device: copy list
     int* state;               // device: blockify state ([0] = counter)
     uint32_t* empty_bits;
     uint8_t* palette;
     uint32_t* pal_bits;
-    unsigned long long stage_blocks;  // byte offset of the staged blocks in the staging buffer
+    long long stage_origin;   // byte offset of the staged origin world, -1: the device copy is current
     unsigned long long stage_copy;    // byte offset of the staged copy list (slots st .. hi - 1)
+    unsigned long long stage_slots;   // byte offset of the staged slot blocks (slots st .. counter - 1)
     int group_begin;          // first workgroup of this world in the commit launch
     int n_blocks;
     int st, hi;               // slots st .. hi - 1 are copied and get their occupancy bits
     int counter;
     int slot_begin;           // first slot of this world in the concatenated st .. hi - 1 ranges
 };
-// blocks + empty-block bits + copy list + counter of every world: world w
-// owns workgroups group_begin .. (next world's group_begin) - 1 of 256
-// threads, one element each.
+// origin blocks + their empty-block bits + copy list + counter of every
+// world: world w owns workgroups group_begin .. (next world's group_begin) - 1
+// of 256 threads, one element each.
 int map_commit_elements(const MapWorldDesc& d);
 hipError_t launch_map_commit_batch(const MapWorldDesc* d_dev, int n, int total_groups, const char* stage_dev,
                                    hipStream_t stream);
-// exec_copies of every world's slots st .. hi - 1.
-hipError_t launch_map_copies_batch(const MapWorldDesc* d_dev, int n, int total_slots, hipStream_t stream);
+// exec_copies of every world's slots st .. hi - 1, and each allocated slot's
+// block (its id, its empty bit cleared).
+hipError_t launch_map_copies_batch(const MapWorldDesc* d_dev, int n, int total_slots, const char* stage_dev,
+                                   hipStream_t stream);
 // Models of many worlds in one launch: model i owns workgroups begin[i] .. begin[i + 1] - 1.
 hipError_t launch_map_batch(const MapArgs* args_dev, const int* begin_dev, int n_models, int total_groups,
                             hipStream_t stream);

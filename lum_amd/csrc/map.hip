@@ -150,9 +150,12 @@ __device__ __forceinline__ int world_of(const MapWorldDesc* d, int n, int b) {
 __global__ void __launch_bounds__(256) map_commit_batch_kernel(const MapWorldDesc* d, int n, const char* stage) {
     const MapWorldDesc& w = d[world_of<&MapWorldDesc::group_begin>(d, n, (int)blockIdx.x)];
     int j = ((int)blockIdx.x - w.group_begin) * 256 + (int)threadIdx.x;
-    const int16_t* sb = (const int16_t*)(stage + w.stage_blocks);
+    const bool staged = w.stage_origin >= 0;
+    const int16_t* sb = staged ? (const int16_t*)(stage + w.stage_origin) : w.origin;
     if (j < w.n_blocks) {
-        w.blocks[j] = sb[j];
+        const int16_t b = sb[j];
+        w.blocks[j] = b;
+        if (staged) w.origin[j] = b;
         return;
     }
     j -= w.n_blocks;
@@ -172,11 +175,18 @@ __global__ void __launch_bounds__(256) map_commit_batch_kernel(const MapWorldDes
     if (j == w.hi - w.st) w.state[0] = w.counter;
 }
 
-// One workgroup of 256 threads per slot, 16 B per thread: exec_copies.
-__global__ void __launch_bounds__(256) map_copies_batch_kernel(const MapWorldDesc* d, int n) {
+// One workgroup of 256 threads per slot, 16 B per thread: exec_copies; the
+// first thread also gives an allocated slot's block its id (after the commit
+// launch wrote the origin there).
+__global__ void __launch_bounds__(256) map_copies_batch_kernel(const MapWorldDesc* d, int n, const char* stage) {
     const int b = (int)blockIdx.x;
     const MapWorldDesc& w = d[world_of<&MapWorldDesc::slot_begin>(d, n, b)];
     const int slot = w.st + (b - w.slot_begin);
+    if (threadIdx.x == 0 && slot < w.counter) {
+        const int blk = ((const int32_t*)(stage + w.stage_slots))[slot - w.st];
+        w.blocks[blk] = (int16_t)slot;
+        atomicAnd(w.empty_bits + (blk >> 5), ~(1u << (blk & 31)));
+    }
     const int src = w.copy_src[slot];
     uint4 v = make_uint4(0u, 0u, 0u, 0u);
     if (src != 0) v = ((const uint4*)(w.palette + ((size_t)src << 12)))[threadIdx.x];
@@ -228,9 +238,10 @@ hipError_t launch_map_commit_batch(const MapWorldDesc* d_dev, int n, int total_g
     return hipGetLastError();
 }
 
-hipError_t launch_map_copies_batch(const MapWorldDesc* d_dev, int n, int total_slots, hipStream_t stream) {
+hipError_t launch_map_copies_batch(const MapWorldDesc* d_dev, int n, int total_slots, const char* stage_dev,
+                                   hipStream_t stream) {
     if (n <= 0 || total_slots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(map_copies_batch_kernel, dim3(total_slots), dim3(256), 0, stream, d_dev, n);
+    hipLaunchKernelGGL(map_copies_batch_kernel, dim3(total_slots), dim3(256), 0, stream, d_dev, n, stage_dev);
     return hipGetLastError();
 }
 
