@@ -663,15 +663,12 @@ __device__ __forceinline__ int world_texels(const RadianceArgs& a) {
 __device__ __forceinline__ int compact_ballots(const RadianceArgs& a, int t, unsigned long long* s_words) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = world_texels(a);
-    int count = 0;
     for (int k = 0; k < kCompactTile / kCompactThreads; ++k) {
         const int p = t * kCompactTile + k * kCompactThreads + (int)threadIdx.x;
         const bool flag = p < n && gld((const int*)a.claim, (size_t)p) >= 0;
         const unsigned long long m = __ballot(flag);
         if (lane == 0) s_words[k * (kCompactThreads / 64) + wave] = m;
-        count += __popcll(m);
     }
-    (void)count;
     __syncthreads();
     int total = 0;
     if (threadIdx.x == 0)
