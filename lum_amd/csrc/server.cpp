@@ -7,8 +7,9 @@
 //      launch from the pending queue in arrival order: a request is taken if
 //      its world is idle, not yet in this launch, no earlier pending request
 //      of the same world was passed over, and the world is (or can be made)
-//      resident within the HBM budget by evicting least-recently-used idle
-//      worlds;
+//      resident within the HBM budget by evicting idle worlds (those the
+//      pending queue names last, then the cheapest, then the least recently
+//      used);
 //   3. the taken radiance requests go into one batched radiance launch and
 //      the glossy requests into one batched glossy launch, on the next stream,
 //      followed by a completion event.
@@ -35,6 +36,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <deque>
 #include <iterator>
 #include <map>
@@ -171,8 +173,9 @@ static long long request_threads(const lumx_request& r, const lumx_world* w) {
 // Make world i resident within the budget (caller holds the lock; `taken`
 // marks worlds already in the launch being built).  The world needs one
 // contiguous arena range: among the windows of that size made only of free
-// space and idle resident worlds, the one whose eviction copies the fewest
-// bytes (then: whose worlds were used least recently) is cleared and taken.
+// space and idle resident worlds, the one whose worlds the pending queue
+// names last (then: whose eviction copies the fewest bytes, then: whose
+// worlds were used least recently) is cleared and taken.
 // Nothing is evicted when no such window exists (busy worlds split the
 // arena); the request then waits.  The restore is queued on the copy stream.
 static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& taken) {
@@ -207,12 +210,25 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
             const ServedWorld& o = s->worlds[g.k];
             return g.k == i || o.busy > 0 || taken[g.k];
         };
+        // Queue position of each world's next pending request: the queue is
+        // the future the server can see, and evicting a world it names soon
+        // only brings that world back.
+        std::vector<size_t> next(s->worlds.size(), SIZE_MAX);
+        size_t pos = 0;
+        for (const Req* r : s->pending) {
+            size_t& nx = next[r->spec.world];
+            if (nx == SIZE_MAX) nx = pos;
+            ++pos;
+        }
+        // Best window: the one whose soonest-needed world is needed last
+        // (none at all is best), then the fewest bytes copied out, then the
+        // least recently used.
         int best = -1;
-        size_t best_cost = 0;
+        size_t best_cost = 0, best_soonest = 0;
         unsigned long long best_age = 0;
         for (int a = 0; a < (int)seg.size(); ++a) {
             if (pinned(seg[a])) continue;
-            size_t got = 0, cost = 0;
+            size_t got = 0, cost = 0, soonest = SIZE_MAX;
             unsigned long long age = 0;
             int b = a;
             for (; b < (int)seg.size() && got < need; ++b) {
@@ -222,12 +238,16 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
                 if (seg[b].k >= 0) {
                     cost += seg[b].size;
                     age = std::max(age, s->worlds[seg[b].k].last_use);
+                    soonest = std::min(soonest, next[seg[b].k]);
                 }
             }
             if (got < need) continue;
-            if (best < 0 || cost < best_cost || (cost == best_cost && age < best_age)) {
+            const bool better = best < 0 || soonest > best_soonest ||
+                                (soonest == best_soonest && (cost < best_cost || (cost == best_cost && age < best_age)));
+            if (better) {
                 best = a;
                 best_cost = cost;
+                best_soonest = soonest;
                 best_age = age;
             }
         }
