@@ -480,9 +480,16 @@ int lumx::world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s
     world_carve(w, (char*)block);
     const int used = w->snap_used;
     const char* h = (const char*)w->snap_host;
-    HIP_TRY(hipMemcpyAsync(block, h, snap_static_bytes(w, used), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(w->field[w->cur], h + snap_field_offset(w, used), (size_t)w->n_blocks * w->texel_bytes,
-                           hipMemcpyHostToDevice, s));
+    const size_t field_bytes = (size_t)w->n_blocks * w->texel_bytes;
+    if ((char*)w->field[w->cur] == (char*)block + snap_field_offset(w, used)) {
+        // Field 0 current and every palette slot in use: the snapshot is the
+        // block's own layout up to the field's end, one copy.
+        HIP_TRY(hipMemcpyAsync(block, h, snap_field_offset(w, used) + field_bytes, hipMemcpyHostToDevice, s));
+    } else {
+        HIP_TRY(hipMemcpyAsync(block, h, snap_static_bytes(w, used), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(w->field[w->cur], h + snap_field_offset(w, used), field_bytes, hipMemcpyHostToDevice,
+                               s));
+    }
     HIP_TRY(world_init_rest(w, used, s));
     return LUMX_OK;
 }
