@@ -117,6 +117,52 @@ def test_batch_compacted_subset_updates_equal_single(gpu_ctx):
             w.close()
 
 
+def test_batch_compacted_updates_in_graph_replays(gpu_ctx):
+    """Compacted subset updates recorded into a graph (claims, compaction,
+    update, scatter) replay to the same fields as the same updates issued
+    eagerly, on every replay, with eager work between replays."""
+    import torch
+    specs = [((16, 16, 16), 4096, 1500, 71), ((24, 20, 10), 3000, 2500, 72)]
+    gw, ew, reqs = [], [], []
+    for size, n, distinct, seed in specs:
+        g, _ = make_world(gpu_ctx, size, seed)
+        e, _ = make_world(gpu_ctx, size, seed)
+        gw.append(g)
+        ew.append(e)
+        wx, wy, wz = size
+        rng = np.random.default_rng(seed + 7)
+        flat = rng.integers(0, distinct, size=n)
+        q = np.stack([flat % wx, (flat // wx) % wy, flat // (wx * wy), np.zeros(n, int)], 1).astype(np.int32)
+        reqs.append(torch.from_numpy(q).cuda())
+    gb, eb = Batch(gpu_ctx, gw), Batch(gpu_ctx, ew)
+    params = [radiance_params(rays=64, time_seed=50 + i, alpha=0.3, max_steps=64) for i in range(len(specs))]
+    stream = torch.cuda.Stream()
+    graph = Graph()
+    try:
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            graph.begin()
+            gb.radiance(params, reqs)
+            graph.end()
+        torch.cuda.synchronize()
+        for rep in range(3):
+            with torch.cuda.stream(stream):
+                graph.launch()
+            eb.radiance(params, reqs)
+            torch.cuda.synchronize()
+            junk = torch.ones(1 << 20, device="cuda") * rep
+            torch.cuda.synchronize()
+            del junk
+            for i in range(len(specs)):
+                assert np.array_equal(gw[i].read_radiance(), ew[i].read_radiance()), (rep, i)
+    finally:
+        graph.close()
+        gb.close()
+        eb.close()
+        for w in gw + ew:
+            w.close()
+
+
 def test_batch_frame_counter_offsets_seed(gpu_ctx):
     import torch
     w, _ = make_world(gpu_ctx, (16, 16, 8), 21)
