@@ -413,7 +413,9 @@ static void world_free_device(lumx_world* w) {
     if (w->dev_own && w->dev_mem) hipFree(w->dev_mem);
     w->dev_mem = nullptr;
     w->dev_own = false;
-    hipFree(w->blockify_scratch);
+    // hipFree waits for the whole device even for a null pointer: an
+    // eviction from the server's arena must not.
+    if (w->blockify_scratch) hipFree(w->blockify_scratch);
     w->blockify_scratch = nullptr;
     w->blockify_scratch_bytes = 0;
     w->blocks = nullptr;
@@ -470,6 +472,16 @@ int lumx::world_snapshot_async(lumx_world* w, hipStream_t s, bool field_only) {
     w->snap_static_valid = true;
     if (w->dev_own) HIP_TRY(hipStreamSynchronize(s));  // the block is freed below
     world_free_device(w);
+    return LUMX_OK;
+}
+
+int lumx::world_snapshot_static(lumx_world* w) {
+    const int used = std::max(w->counter, w->prev_counter);
+    int rc = world_snapshot_reserve(w, used, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(w->snap_host, w->dev_mem, snap_static_bytes(w, used), hipMemcpyDeviceToHost));
+    w->snap_used = used;
+    w->snap_static_valid = true;
     return LUMX_OK;
 }
 
@@ -1172,7 +1184,7 @@ static int map_device(lumx_world_t w, int n, const lumx_model_t* models, const l
             return fail(LUMX_EINVAL, "device map scratch too small inside a capture: run the same map once "
                                      "outside the capture first");
         HIP_TRY(hipStreamSynchronize(s));
-        hipFree(w->blockify_scratch);
+        if (w->blockify_scratch) hipFree(w->blockify_scratch);
         w->blockify_scratch = nullptr;
         w->blockify_scratch_bytes = 0;
         HIP_TRY(hipMalloc(&w->blockify_scratch, need));

@@ -157,6 +157,9 @@ size_t world_bytes(const lumx_world* w);
 //                           fills it from the snapshot on s.
 int world_snapshot_reserve(lumx_world* w, int used, hipStream_t s);
 int world_snapshot_async(lumx_world* w, hipStream_t s, bool field_only);
+// The static state into the snapshot now (synchronous; the world stays
+// resident), so later evictions copy only the field.
+int world_snapshot_static(lumx_world* w);
 int world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s);
 // The world's device parts now live in `block` (a copy of its old block).
 void world_rebase(lumx_world* w, void* block, bool own);

@@ -633,6 +633,11 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
             }
             if (w->dev_own) hipFree(w->dev_mem);
             world_rebase(w, s->arena.base + off, false);
+            // Its static state into the snapshot now: evictions while serving
+            // then copy only the field (a first copy into a pinned snapshot
+            // has been seen to hold the host for milliseconds).
+            rc = world_snapshot_static(w);
+            if (rc) return rc;
         } else {
             rc = lumx_world_evict(w);
             if (rc) return rc;
@@ -641,8 +646,8 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
     ServedWorld sw;
     sw.w = w;
     if (resident(w) && !w->dev_own) sw.off = (size_t)((char*)w->dev_mem - s->arena.base);
-    // A world on the host holds a full snapshot.
-    sw.static_kept = s->desc.hbm_budget != 0 && !resident(w) && w->snap_static_valid;
+    // A world on the host holds a full snapshot; one in the arena, its static state.
+    sw.static_kept = s->desc.hbm_budget != 0 && w->snap_static_valid;
     HIP_TRY(hipEventCreateWithFlags(&sw.ready, hipEventDisableTiming));
     s->worlds.push_back(sw);
     if (resident(w)) s->resident_bytes += w->device_bytes;
