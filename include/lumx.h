@@ -367,8 +367,8 @@ int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
  * counterpart: the reference renders one world per frame) ----
  * A worker thread admits submitted requests in arrival order, at most one
  * per world per launch and in submission order per world, restoring evicted
- * worlds (and evicting least-recently-used idle ones) to stay within the HBM
- * budget, packs up to max_batch requests into one batched radiance launch
+ * worlds (and evicting the idle ones the pending requests need last) to stay
+ * within the HBM budget, packs up to max_batch requests into one batched radiance launch
  * plus one batched glossy launch, and keeps up to `streams` launches in
  * flight while it admits the next.  Each request computes exactly what its
  * own lumx_radiance_update / lumx_glossy call would.  Worlds added to a
@@ -376,8 +376,10 @@ int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
  * With an HBM budget the server allocates one device arena of the budget's
  * size at creation and every world lives there or in a pinned host snapshot:
  * lumx_server_add_world moves a resident world into the arena while it has
- * room, else to the host; a world larger than the budget is LUMX_EINVAL.  Evictions and restores are copies on the
- * server's own stream, never a host wait.  At destroy, worlds in the arena
+ * room (copying its static state to its snapshot, so evictions while serving
+ * copy only its field), else to the host; a world larger than the budget is
+ * LUMX_EINVAL.  Evictions and restores are copies on the server's own stream,
+ * never a host wait.  At destroy, worlds in the arena
  * move into device blocks of their own; evicted ones stay on the host
  * (lumx_world_restore brings them back). */
 #define LUMX_REQ_RADIANCE 0
