@@ -263,6 +263,25 @@ def test_radiance_compacted_update_bit_exact(gpu_ctx, tuning, rays, fmt):
         assert np.array_equal(results[0], ref)
 
 
+@pytest.mark.parametrize("tuning", ["lane", "lane_rpl4"])
+def test_radiance_compacted_strided_update_bit_exact(gpu_ctx, tuning):
+    """A compacted update without a request list (every fourth probe, from
+    the third, in x-fastest order) gives the oracle's field bit for bit."""
+    size = (16, 16, 32)
+    blocks, pal, mats, field = make_case(size, 311)
+    w = gpu_world(gpu_ctx, blocks, pal, mats, field)
+    w.set_tuning(**dict(TUNINGS[tuning], compact=1))
+    full = full_palette(w, pal)
+    try:
+        w.radiance_update(None, radiance_params(rays=64, time_seed=31, size=4, shift=2, alpha=0.3))
+        got = w.read_radiance()
+    finally:
+        w.close()
+    wd = orc.WorldData(blocks, full, mats)
+    ref = orc.radiance_update(wd, field, rays=64, time_seed=31, size=4, shift=2, alpha=0.3)
+    assert np.array_equal(got, ref)
+
+
 def test_request_list_longer_than_world_rejected(gpu_ctx):
     """A list dispatching more probes than the world holds is refused before
     any launch: the staged texels live in one texel per probe."""
