@@ -239,8 +239,11 @@ int launcher_radiance(Launcher* L, lumx_world* const* worlds, int n, const int32
     // Subset updates land in their current fields in place.  A trial launch
     // (commit = false) writes the free field buffers only.
     if (commit) {
-        HIP_TRY(launch_scatter_batch(args_dev, (const int*)((char*)slot->dev + args_bytes + begin_bytes), n,
-                                     (int)total_staged, (int)worlds[0]->texel_bytes, s));
+        bool listed = false;  // a staged world that is not compacted
+        for (int i = 0; i < n && !listed; ++i) listed = args[i].staged && !args[i].compact;
+        if (listed)
+            HIP_TRY(launch_scatter_batch(args_dev, (const int*)((char*)slot->dev + args_bytes + begin_bytes), n,
+                                         (int)total_staged, (int)worlds[0]->texel_bytes, s));
         HIP_TRY(launch_compact_scatter_batch(args_dev, tbegin_dev, n, (int)total_tiles,
                                              (int)worlds[0]->texel_bytes, s));
         for (int i = 0; i < n; ++i)
