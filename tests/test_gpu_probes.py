@@ -103,9 +103,11 @@ def test_device_selection_follows_map(gpu_ctx):
         w.close()
 
 
-@pytest.mark.parametrize("tuning", ["lane", "lane_occ", "tiled"])
+@pytest.mark.parametrize("tuning", ["lane", "lane_occ", "tiled", "lane_compacted"])
 @pytest.mark.parametrize("size_shift", [(1, 0), (3, 2)])
 def test_counted_update_equals_listed_update(gpu_ctx, tuning, size_shift):
+    """lane_compacted: the counted update runs compacted (claimed probes in
+    texel order, the count read on the device), the listed one never does."""
     import torch
     size = (24, 24, 12)
     blocks, pal, mats, field = make_case(size, 31)
@@ -114,8 +116,12 @@ def test_counted_update_equals_listed_update(gpu_ctx, tuning, size_shift):
     a = gpu_world(gpu_ctx, blocks, pal, mats, field)
     b = gpu_world(gpu_ctx, blocks, pal, mats, field)
     try:
-        a.set_tuning(**TUNINGS[tuning])
-        b.set_tuning(**TUNINGS[tuning])
+        if tuning == "lane_compacted":
+            a.set_tuning(**dict(TUNINGS["lane"], compact=1))
+            b.set_tuning(**dict(TUNINGS["lane"], compact=-1))
+        else:
+            a.set_tuning(**TUNINGS[tuning])
+            b.set_tuning(**TUNINGS[tuning])
         p = radiance_params(rays=64, time_seed=5, alpha=0.2, max_steps=64, size=size_shift[0],
                             shift=size_shift[1])
         listed, req, cnt = device_select(a, special)
@@ -129,14 +135,19 @@ def test_counted_update_equals_listed_update(gpu_ctx, tuning, size_shift):
         b.close()
 
 
-def test_counted_update_clamps_to_capacity(gpu_ctx):
-    """A device count above the list's capacity dispatches the capacity."""
+@pytest.mark.parametrize("compact", [0, 1])
+def test_counted_update_clamps_to_capacity(gpu_ctx, compact):
+    """A device count above the list's capacity dispatches the capacity
+    (compact=1: the counted update compacted, the listed one not)."""
     import torch
     size = (16, 16, 8)
     blocks, pal, mats, field = make_case(size, 41)
     a = gpu_world(gpu_ctx, blocks, pal, mats, field)
     b = gpu_world(gpu_ctx, blocks, pal, mats, field)
     try:
+        if compact:
+            a.set_tuning(**dict(TUNINGS["lane"], compact=1))
+            b.set_tuning(**dict(TUNINGS["lane"], compact=-1))
         req = torch.from_numpy(orc.select_probes(blocks, None)[:100].copy()).cuda()
         cnt = torch.tensor([1 << 30], dtype=torch.int32, device="cuda")
         p = radiance_params(rays=32, time_seed=1, alpha=0.3, max_steps=64)
