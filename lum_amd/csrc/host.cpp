@@ -485,7 +485,7 @@ int lumx::world_snapshot_static(lumx_world* w) {
     return LUMX_OK;
 }
 
-int lumx::world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s) {
+int lumx::world_attach_copy_async(lumx_world* w, void* block, bool own, hipStream_t s) {
     if (!w->snap_host) return fail(LUMX_EINVAL, "world has no snapshot to restore");
     w->dev_mem = block;
     w->dev_own = own;
@@ -493,17 +493,30 @@ int lumx::world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s
     const int used = w->snap_used;
     const char* h = (const char*)w->snap_host;
     const size_t field_bytes = (size_t)w->n_blocks * w->texel_bytes;
-    if ((char*)w->field[w->cur] == (char*)block + snap_field_offset(w, used)) {
-        // Field 0 current and every palette slot in use: the snapshot is the
-        // block's own layout up to the field's end, one copy.
+    if ((char*)w->field[0] == (char*)block + snap_field_offset(w, used)) {
+        // Every palette slot in use: the snapshot is the block's own layout
+        // up to field 0's end, so one copy, and the field comes back as
+        // field 0 (which of the two buffers holds it is the world's own
+        // bookkeeping: the other is zeroed by world_attach_init).
         HIP_TRY(hipMemcpyAsync(block, h, snap_field_offset(w, used) + field_bytes, hipMemcpyHostToDevice, s));
+        w->cur = 0;
     } else {
         HIP_TRY(hipMemcpyAsync(block, h, snap_static_bytes(w, used), hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(w->field[w->cur], h + snap_field_offset(w, used), field_bytes, hipMemcpyHostToDevice,
                                s));
     }
-    HIP_TRY(world_init_rest(w, used, s));
     return LUMX_OK;
+}
+
+int lumx::world_attach_init(lumx_world* w, hipStream_t s) {
+    HIP_TRY(world_init_rest(w, w->snap_used, s));
+    return LUMX_OK;
+}
+
+int lumx::world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s) {
+    int rc = world_attach_copy_async(w, block, own, s);
+    if (rc) return rc;
+    return world_attach_init(w, s);
 }
 }  // extern "C++"
 

@@ -154,13 +154,21 @@ size_t world_bytes(const lumx_world* w);
 //                           sync of s when it is its own; a lent block is the
 //                           lender's again);
 //   world_attach_async      the world takes `block` (its own when `own`) and
-//                           fills it from the snapshot on s.
+//                           fills it from the snapshot on s: the copies
+//                           (world_attach_copy_async), then one kernel that
+//                           sets up the rest of the block (world_attach_init:
+//                           the other field and unused slots zeroed, claims
+//                           idle, counters, occupancy bits), which may run
+//                           later on another stream ordered after the copies
+//                           and before any other use of the world.
 int world_snapshot_reserve(lumx_world* w, int used, hipStream_t s);
 int world_snapshot_async(lumx_world* w, hipStream_t s, bool field_only);
 // The static state into the snapshot now (synchronous; the world stays
 // resident), so later evictions copy only the field.
 int world_snapshot_static(lumx_world* w);
 int world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s);
+int world_attach_copy_async(lumx_world* w, void* block, bool own, hipStream_t s);
+int world_attach_init(lumx_world* w, hipStream_t s);
 // The world's device parts now live in `block` (a copy of its old block).
 void world_rebase(lumx_world* w, void* block, bool own);
 

@@ -72,6 +72,7 @@ struct ServedWorld {
     bool static_kept = false;  // its snapshot holds its static state, taken in this server
     hipEvent_t ready = nullptr;  // recorded on the copy stream after its last restore
     bool ready_pending = false;  // no launch has waited for that restore yet
+    bool init_pending = false;   // restored, the rest of its block not yet set up
 };
 
 struct Flight {
@@ -148,6 +149,9 @@ struct lumx_server {
     std::vector<std::unique_ptr<Req>> tickets;
     std::vector<Flight> flights;
     bool stop = false;
+    // Something changed since prefetch last ran (a launch retired or was
+    // queued, a request arrived): what it could not restore then may fit now.
+    bool prefetch_ready = true;
     unsigned long long use_clock = 0;
     size_t resident_bytes = 0;
     lumx_server_stats stats{};
@@ -170,15 +174,30 @@ static long long request_threads(const lumx_request& r, const lumx_world* w) {
     return (long long)((r.width + 15) / 16) * ((r.height + 15) / 16) * 256;
 }
 
+// Queue position of each world's next pending request (SIZE_MAX: none): the
+// queue is the future the server can see, and evicting a world it names soon
+// only brings that world back.  Taken once per launch being built: requests
+// leaving the queue meanwhile belong to worlds that launch pins anyway.
+static std::vector<size_t> next_use(const lumx_server* s) {
+    std::vector<size_t> next(s->worlds.size(), SIZE_MAX);
+    size_t pos = 0;
+    for (const Req* r : s->pending) {
+        size_t& nx = next[r->spec.world];
+        if (nx == SIZE_MAX) nx = pos;
+        ++pos;
+    }
+    return next;
+}
+
 // Make world i resident within the budget (caller holds the lock; `taken`
-// marks worlds already in the launch being built).  The world needs one
-// contiguous arena range: among the windows of that size made only of free
-// space and idle resident worlds, the one whose worlds the pending queue
-// names last (then: whose eviction copies the fewest bytes, then: whose
-// worlds were used least recently) is cleared and taken.
+// marks worlds already in the launch being built, `next` is next_use()).
+// The world needs one contiguous arena range: among the windows of that size
+// made only of free space and idle resident worlds, the one whose worlds the
+// pending queue names last (then: whose eviction copies the fewest bytes,
+// then: whose worlds were used least recently) is cleared and taken.
 // Nothing is evicted when no such window exists (busy worlds split the
 // arena); the request then waits.  The restore is queued on the copy stream.
-static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& taken) {
+static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& taken, const std::vector<size_t>& next) {
     ServedWorld& sw = s->worlds[i];
     if (resident(sw.w)) return true;
     const size_t need = sw.w->device_bytes;
@@ -210,16 +229,6 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
             const ServedWorld& o = s->worlds[g.k];
             return g.k == i || o.busy > 0 || taken[g.k];
         };
-        // Queue position of each world's next pending request: the queue is
-        // the future the server can see, and evicting a world it names soon
-        // only brings that world back.
-        std::vector<size_t> next(s->worlds.size(), SIZE_MAX);
-        size_t pos = 0;
-        for (const Req* r : s->pending) {
-            size_t& nx = next[r->spec.world];
-            if (nx == SIZE_MAX) nx = pos;
-            ++pos;
-        }
         // Best window: the one whose soonest-needed world is needed last
         // (none at all is best), then the fewest bytes copied out, then the
         // least recently used.
@@ -258,8 +267,11 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
             ServedWorld& v = s->worlds[seg[b].k];
             // Serving changes only fields: after the world's first snapshot in
             // this server only its field goes out.
+            // (A world restored ahead of a launch that never ran sends back
+            // the field it received: the set-up it skipped is not needed.)
             if (world_snapshot_async(v.w, s->copy_stream, v.static_kept) != LUMX_OK) return false;
             v.static_kept = true;
+            v.init_pending = false;
             s->arena.give(v.off, v.w->device_bytes);
             s->resident_bytes -= v.w->device_bytes;
             s->stats.evictions++;
@@ -267,13 +279,23 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
         if (!s->arena.take_at(start, need)) return false;
         off = start;
     }
-    if (world_attach_async(sw.w, s->arena.base + off, false, s->copy_stream) != LUMX_OK) {
+    // Batched engine: only the copies go on the copy stream; the kernel that
+    // sets up the rest of the block runs on the stream of the world's next
+    // launch, so the copy stream carries copies alone and no launch waits
+    // behind another world's set-up.  The direct engine's one launch stream
+    // runs its calls back to back, so there the set-up stays on the copy
+    // stream, beside them.
+    const bool defer = !s->desc.direct;
+    const int rc = defer ? world_attach_copy_async(sw.w, s->arena.base + off, false, s->copy_stream)
+                         : world_attach_async(sw.w, s->arena.base + off, false, s->copy_stream);
+    if (rc != LUMX_OK) {
         s->arena.give(off, need);
         return false;
     }
     sw.off = off;
     s->resident_bytes += need;
     s->stats.restores++;
+    sw.init_pending = defer;
     // Launches of this world wait for its restore (and, the copy stream being
     // in order, for the evictions queued before it).
     if (hipEventRecord(sw.ready, s->copy_stream) != hipSuccess) return false;
@@ -281,13 +303,22 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
     return true;
 }
 
-// Launches on `st` of flight f start after its worlds' restores.
+// Launches on `st` of flight f start after its worlds' restores, and a
+// restored world's block is set up on `st` first.  A world is in one flight
+// at a time (the direct engine's calls share one stream), so whatever uses
+// the world next is ordered after that set-up.
 static int order_after_restores(lumx_server* s, const Flight& f, hipStream_t st) {
     for (int wi : f.worlds) {
         ServedWorld& sw = s->worlds[wi];
-        if (!sw.ready_pending) continue;
-        HIP_TRY(hipStreamWaitEvent(st, sw.ready, 0));
-        sw.ready_pending = false;
+        if (sw.ready_pending) {
+            HIP_TRY(hipStreamWaitEvent(st, sw.ready, 0));
+            sw.ready_pending = false;
+        }
+        if (sw.init_pending) {
+            int rc = world_attach_init(sw.w, st);
+            if (rc) return rc;
+            sw.init_pending = false;
+        }
     }
     return LUMX_OK;
 }
@@ -301,11 +332,12 @@ static void prefetch(lumx_server* s) {
     int n = 0;
     for (auto it = s->pending.begin(); it != s->pending.end() && n < s->desc.max_batch; ++it, ++n)
         keep[(*it)->spec.world] = 1;
+    const std::vector<size_t> next = next_use(s);
     n = 0;
     for (auto it = s->pending.begin(); it != s->pending.end() && n < s->desc.max_batch; ++it, ++n) {
         const int wi = (*it)->spec.world;
         if (resident(s->worlds[wi].w)) continue;
-        if (!ensure_resident(s, wi, keep)) break;
+        if (!ensure_resident(s, wi, keep, next)) break;
     }
 }
 
@@ -406,7 +438,10 @@ static void worker_loop(lumx_server* s) {
                 ++k;
             }
         }
-        if (retired) s->cv_done.notify_all();
+        if (retired) {
+            s->prefetch_ready = true;
+            s->cv_done.notify_all();
+        }
         if (s->stop && s->pending.empty() && s->flights.empty()) break;
 
         // 2. build the next launch
@@ -415,6 +450,11 @@ static void worker_loop(lumx_server* s) {
         if ((int)s->flights.size() < depth && !s->pending.empty()) {
             const int nw = (int)s->worlds.size();
             std::vector<char> taken(nw, 0), blocked(nw, 0);
+            // Windows only shrink while a launch is built (its worlds are
+            // pinned as they join it): once a world of `no_room` bytes found
+            // none, no larger world will.
+            std::vector<size_t> next;
+            size_t no_room = SIZE_MAX;
             Flight f;
             long long threads[2] = {0, 0};  // radiance, glossy threads of the launch being built
             for (auto it = s->pending.begin(); it != s->pending.end() && (int)f.reqs.size() < s->desc.max_batch;) {
@@ -423,7 +463,8 @@ static void worker_loop(lumx_server* s) {
                 if (s->desc.direct) {
                     // Arrival order, one request per call; a world's calls
                     // are ordered by the stream, so it need not be idle.
-                    if (!f.reqs.empty() || !ensure_resident(s, wi, taken)) break;
+                    if (!f.reqs.empty()) break;
+                    if (!resident(s->worlds[wi].w) && !ensure_resident(s, wi, taken, next_use(s))) break;
                     s->worlds[wi].busy++;
                     s->worlds[wi].last_use = ++s->use_clock;
                     f.reqs.push_back(r);
@@ -443,7 +484,16 @@ static void worker_loop(lumx_server* s) {
                     // its world wait for the next launch.
                     ok = false;
                 }
-                if (ok) ok = ensure_resident(s, wi, taken);
+                if (ok && !resident(s->worlds[wi].w)) {
+                    const size_t need = s->worlds[wi].w->device_bytes;
+                    if (need >= no_room) {
+                        ok = false;
+                    } else {
+                        if (next.empty()) next = next_use(s);
+                        ok = ensure_resident(s, wi, taken, next);
+                        if (!ok) no_room = need;
+                    }
+                }
                 if (!ok) {
                     blocked[wi] = 1;
                     ++it;
@@ -494,9 +544,15 @@ static void worker_loop(lumx_server* s) {
                 launched = true;
             }
         }
-        if (launched) continue;
-        if (!s->desc.direct && s->desc.hbm_budget && (int)s->flights.size() >= depth && !s->pending.empty())
+        if (launched) {
+            s->prefetch_ready = true;
+            continue;
+        }
+        if (!s->desc.direct && s->desc.hbm_budget && (int)s->flights.size() >= depth && !s->pending.empty() &&
+            s->prefetch_ready) {
+            s->prefetch_ready = false;
             prefetch(s);
+        }
         // 3. wait: poll in-flight events briefly, or sleep until a submission
         if (!s->flights.empty()) {
             lk.unlock();
@@ -575,8 +631,14 @@ int lumx_server_destroy(lumx_server_t s) {
         hipStreamSynchronize(t);
         hipStreamDestroy(t);
     }
-    hipStreamSynchronize(s->copy_stream);
     int status = LUMX_OK;
+    // Worlds restored for launches that never ran: their blocks set up now.
+    for (ServedWorld& sw : s->worlds) {
+        if (!sw.init_pending || !resident(sw.w)) continue;
+        if (world_attach_init(sw.w, s->copy_stream) != LUMX_OK) status = LUMX_EDEVICE;
+        sw.init_pending = false;
+    }
+    hipStreamSynchronize(s->copy_stream);
     if (s->arena.base) {
         // Worlds in the arena move into blocks of their own (the same layout,
         // so one device copy each); a world that cannot stays on the host.
@@ -687,6 +749,7 @@ int lumx_server_submit(lumx_server_t s, const lumx_request* r, long long* ticket
         q->t_submit = Clock::now();
         *ticket = (long long)s->tickets.size();
         s->pending.push_back(q.get());
+        s->prefetch_ready = true;
         s->tickets.push_back(std::move(q));
         s->stats.submitted++;
     }
