@@ -367,7 +367,9 @@ void lumx::world_rebase(lumx_world* w, void* block, bool own) {
 // other field and the palette slots above `used` zeroed, claims idle,
 // counters zero (the map counter `counter`), occupancy bits rebuilt from the
 // blocks and the palette (one kernel on s).
-static hipError_t world_init_rest(lumx_world* w, int used, hipStream_t s) {
+// What world_attach_kernel needs to set up the rest of w's block, palette
+// slots below `used` kept (the next map re-uploads the origin world).
+static lumx::AttachArgs attach_args(lumx_world* w, int used) {
     lumx::AttachArgs a{};
     a.field_other = (uint2*)w->field[w->cur ^ 1];
     a.n_field_words = (long long)w->n_blocks * (long long)(w->texel_bytes / 8);
@@ -385,7 +387,11 @@ static hipError_t world_init_rest(lumx_world* w, int used, hipStream_t s) {
     a.blockify_state = w->blockify_state;
     a.counter = w->counter;
     w->origin_dirty = true;
-    return lumx::launch_world_attach(a, s);
+    return a;
+}
+
+static hipError_t world_init_rest(lumx_world* w, int used, hipStream_t s) {
+    return lumx::launch_world_attach(attach_args(w, used), s);
 }
 
 // A world's own device block, every part zeroed (claims and counters idle).
@@ -510,6 +516,13 @@ int lumx::world_attach_copy_async(lumx_world* w, void* block, bool own, hipStrea
 
 int lumx::world_attach_init(lumx_world* w, hipStream_t s) {
     HIP_TRY(world_init_rest(w, w->snap_used, s));
+    return LUMX_OK;
+}
+
+int lumx::world_attach_init_many(lumx_world* const* ws, int n, hipStream_t s) {
+    std::vector<lumx::AttachArgs> a((size_t)n);
+    for (int i = 0; i < n; ++i) a[i] = attach_args(ws[i], ws[i]->snap_used);
+    HIP_TRY(lumx::launch_world_attach_batch(a.data(), n, s));
     return LUMX_OK;
 }
 

@@ -169,6 +169,8 @@ int world_snapshot_static(lumx_world* w);
 int world_attach_async(lumx_world* w, void* block, bool own, hipStream_t s);
 int world_attach_copy_async(lumx_world* w, void* block, bool own, hipStream_t s);
 int world_attach_init(lumx_world* w, hipStream_t s);
+// world_attach_init of n worlds, in launches of up to 16 worlds.
+int world_attach_init_many(lumx_world* const* ws, int n, hipStream_t s);
 // The world's device parts now live in `block` (a copy of its old block).
 void world_rebase(lumx_world* w, void* block, bool own);
 

@@ -208,6 +208,9 @@ struct AttachArgs {
     int counter;                // blockify_state[0]
 };
 hipError_t launch_world_attach(const AttachArgs& a, hipStream_t stream);
+// Worlds a[0..n) set up by launches of up to kAttachBatch worlds each.
+constexpr int kAttachBatch = 16;
+hipError_t launch_world_attach_batch(const AttachArgs* a, int n, hipStream_t stream);
 
 size_t select_tiles(int n_probes);
 size_t select_bits_bytes(int n_probes);

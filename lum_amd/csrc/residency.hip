@@ -1,7 +1,8 @@
 // residency.hip - the device half of bringing a world into a device block
 // (restore from its host snapshot, or creation): after the kept state (current
 // field, blocks, copy list, materials, palette slots in use) has been copied
-// in, one kernel sets everything else: the other field and the palette slots
+// in, one kernel (for one world, or for up to kAttachBatch worlds at once)
+// sets everything else: the other field and the palette slots
 // above the ones in use to zero, the claim words idle, the counters, and the
 // occupancy bits of the blocks and of the palette slots rebuilt from the data.
 #include <algorithm>
@@ -12,16 +13,17 @@ namespace lumx {
 
 namespace {
 
-// One grid-stride pass over the concatenation of five index ranges.
-__global__ __launch_bounds__(256) void world_attach_kernel(AttachArgs a) {
+// One world's pass over the concatenation of five index ranges: elements
+// first, first + stride, ...; `lead` (one thread of the world's threads)
+// also resets the counters.
+__device__ __forceinline__ void attach_world(const AttachArgs& a, long long first, long long stride, bool lead) {
     const long long n_field = a.n_field_words;
     const long long n_pal = (long long)(a.slots - a.used) * 512;  // uint2 words of the unused slots
     const long long n_claim = a.n_blocks;
     const long long n_eb = (a.n_blocks + 31) / 32;
     const long long n_pb = (long long)a.slots * 128;
     const long long total = n_field + n_pal + n_claim + n_eb + n_pb;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (lead) {
         a.batch_counter[0] = 0;
         a.batch_counter[1] = 0;
         a.blockify_state[0] = a.counter;
@@ -29,7 +31,7 @@ __global__ __launch_bounds__(256) void world_attach_kernel(AttachArgs a) {
         a.blockify_state[2] = 0;
         a.blockify_state[3] = 0;
     }
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    for (long long i = first; i < total; i += stride) {
         long long j = i;
         if (j < n_field) {
             a.field_other[j] = make_uint2(0u, 0u);
@@ -69,14 +71,57 @@ __global__ __launch_bounds__(256) void world_attach_kernel(AttachArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void world_attach_kernel(AttachArgs a) {
+    attach_world(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x,
+                 blockIdx.x == 0 && threadIdx.x == 0);
+}
+
+// Up to kAttachBatch worlds in one launch, their descriptors passed by value:
+// world k owns workgroups k * per .. k * per + per - 1.
+struct AttachBatch {
+    AttachArgs w[kAttachBatch];
+    int n, per;
+};
+
+static_assert(sizeof(AttachBatch) <= 4096, "kernel arguments are limited to 4 KiB");
+
+__global__ __launch_bounds__(256) void world_attach_batch_kernel(AttachBatch b) {
+    const int k = (int)blockIdx.x / b.per;
+    if (k >= b.n) return;
+    const int g = (int)blockIdx.x - k * b.per;
+    attach_world(b.w[k], (long long)g * blockDim.x + threadIdx.x, (long long)b.per * blockDim.x,
+                 g == 0 && threadIdx.x == 0);
+}
+
+// Workgroups of one world's pass (each thread then takes a grid stride).
+long long attach_groups(const AttachArgs& a) {
+    const long long total = a.n_field_words + (long long)(a.slots - a.used) * 512 + a.n_blocks +
+                            (a.n_blocks + 31) / 32 + (long long)a.slots * 128;
+    return std::max<long long>(std::min<long long>((total + 255) / 256, 2048), 1);
+}
+
 }  // namespace
 
 hipError_t launch_world_attach(const AttachArgs& a, hipStream_t stream) {
-    const long long total = a.n_field_words + (long long)(a.slots - a.used) * 512 + a.n_blocks +
-                            (a.n_blocks + 31) / 32 + (long long)a.slots * 128;
-    const long long groups = std::min<long long>((total + 255) / 256, 2048);
-    hipLaunchKernelGGL(world_attach_kernel, dim3((unsigned)std::max<long long>(groups, 1)), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(world_attach_kernel, dim3((unsigned)attach_groups(a)), dim3(256), 0, stream, a);
     return hipGetLastError();
+}
+
+hipError_t launch_world_attach_batch(const AttachArgs* a, int n, hipStream_t stream) {
+    for (int first = 0; first < n; first += kAttachBatch) {
+        AttachBatch b{};
+        b.n = std::min(kAttachBatch, n - first);
+        long long per = 1;
+        for (int k = 0; k < b.n; ++k) {
+            b.w[k] = a[first + k];
+            per = std::max(per, attach_groups(b.w[k]));
+        }
+        b.per = (int)per;  // <= 2048, so the grid stays below 2^15 workgroups
+        hipLaunchKernelGGL(world_attach_batch_kernel, dim3((unsigned)(b.n * b.per)), dim3(256), 0, stream, b);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace lumx

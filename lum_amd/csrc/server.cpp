@@ -308,18 +308,20 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
 // at a time (the direct engine's calls share one stream), so whatever uses
 // the world next is ordered after that set-up.
 static int order_after_restores(lumx_server* s, const Flight& f, hipStream_t st) {
+    std::vector<lumx_world*> init;
     for (int wi : f.worlds) {
         ServedWorld& sw = s->worlds[wi];
         if (sw.ready_pending) {
             HIP_TRY(hipStreamWaitEvent(st, sw.ready, 0));
             sw.ready_pending = false;
         }
-        if (sw.init_pending) {
-            int rc = world_attach_init(sw.w, st);
-            if (rc) return rc;
-            sw.init_pending = false;
-        }
+        if (sw.init_pending) init.push_back(sw.w);
     }
+    if (init.empty()) return LUMX_OK;
+    // One set-up launch for all of them.
+    int rc = world_attach_init_many(init.data(), (int)init.size(), st);
+    if (rc) return rc;
+    for (int wi : f.worlds) s->worlds[wi].init_pending = false;
     return LUMX_OK;
 }
 
