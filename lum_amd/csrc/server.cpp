@@ -72,6 +72,7 @@ struct ServedWorld {
     bool static_kept = false;  // its snapshot holds its static state, taken in this server
     hipEvent_t ready = nullptr;  // recorded on the copy stream after its last restore
     bool ready_pending = false;  // no launch has waited for that restore yet
+    unsigned long long ready_seq = 0;  // copy-stream order of that restore
     bool init_pending = false;   // restored, the rest of its block not yet set up
 };
 
@@ -132,6 +133,12 @@ struct Arena {
 
 }  // namespace
 
+// A run of the arena in address order: a free gap (k = -1) or resident world k.
+struct ArenaSeg {
+    size_t off, size;
+    int k;
+};
+
 struct lumx_server {
     lumx_context* ctx = nullptr;
     lumx_server_desc desc{};
@@ -140,7 +147,10 @@ struct lumx_server {
     std::vector<hipEvent_t> free_events;
     int next_stream = 0;
     hipStream_t copy_stream = nullptr;  // evictions and restores
+    unsigned long long copy_seq = 0;    // restores recorded on it so far
     Arena arena;
+    std::vector<ArenaSeg> segs;  // the arena as runs, while segs_valid
+    bool segs_valid = false;     // cleared by every change of the arena
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -210,21 +220,26 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
         return true;
     }
     size_t off = 0;
-    if (!s->arena.take(need, &off)) {
+    if (s->arena.take(need, &off)) {
+        s->segs_valid = false;
+    } else {
         // The arena as segments in address order: free gaps (k = -1) and
-        // resident worlds; pinned ones (busy, taken) end a window.
-        struct Seg {
-            size_t off, size;
-            int k;
-        };
-        std::vector<Seg> seg;
-        for (const auto& f : s->arena.free) seg.push_back({f.first, f.second, -1});
-        for (int k = 0; k < (int)s->worlds.size(); ++k) {
-            const ServedWorld& o = s->worlds[k];
-            if (resident(o.w) && !o.w->dev_own) seg.push_back({o.off, o.w->device_bytes, k});
+        // resident worlds; pinned ones (busy, taken) end a window.  Kept
+        // between calls while the arena is unchanged (a launch being built
+        // asks for many worlds that do not fit).
+        if (!s->segs_valid) {
+            s->segs.clear();
+            for (const auto& f : s->arena.free) s->segs.push_back({f.first, f.second, -1});
+            for (int k = 0; k < (int)s->worlds.size(); ++k) {
+                const ServedWorld& o = s->worlds[k];
+                if (resident(o.w) && !o.w->dev_own) s->segs.push_back({o.off, o.w->device_bytes, k});
+            }
+            std::sort(s->segs.begin(), s->segs.end(),
+                      [](const ArenaSeg& x, const ArenaSeg& y) { return x.off < y.off; });
+            s->segs_valid = true;
         }
-        std::sort(seg.begin(), seg.end(), [](const Seg& x, const Seg& y) { return x.off < y.off; });
-        auto pinned = [&](const Seg& g) {
+        const std::vector<ArenaSeg>& seg = s->segs;
+        auto pinned = [&](const ArenaSeg& g) {
             if (g.k < 0) return false;
             const ServedWorld& o = s->worlds[g.k];
             return g.k == i || o.busy > 0 || taken[g.k];
@@ -262,9 +277,11 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
         }
         if (best < 0) return false;
         const size_t start = seg[best].off;
-        for (int b = best; b < (int)seg.size() && seg[b].off < start + need; ++b) {
-            if (seg[b].k < 0) continue;
-            ServedWorld& v = s->worlds[seg[b].k];
+        const std::vector<ArenaSeg> window(seg.begin() + best, seg.end());
+        s->segs_valid = false;
+        for (int b = 0; b < (int)window.size() && window[b].off < start + need; ++b) {
+            if (window[b].k < 0) continue;
+            ServedWorld& v = s->worlds[window[b].k];
             // Serving changes only fields: after the world's first snapshot in
             // this server only its field goes out.
             // (A world restored ahead of a launch that never ran sends back
@@ -290,6 +307,7 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
                          : world_attach_async(sw.w, s->arena.base + off, false, s->copy_stream);
     if (rc != LUMX_OK) {
         s->arena.give(off, need);
+        s->segs_valid = false;
         return false;
     }
     sw.off = off;
@@ -300,6 +318,7 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
     // in order, for the evictions queued before it).
     if (hipEventRecord(sw.ready, s->copy_stream) != hipSuccess) return false;
     sw.ready_pending = true;
+    sw.ready_seq = ++s->copy_seq;
     return true;
 }
 
@@ -308,15 +327,17 @@ static bool ensure_resident(lumx_server* s, int i, const std::vector<char>& take
 // at a time (the direct engine's calls share one stream), so whatever uses
 // the world next is ordered after that set-up.
 static int order_after_restores(lumx_server* s, const Flight& f, hipStream_t st) {
+    // The copy stream runs in order: waiting for the latest of the flight's
+    // restores waits for all of them.
     std::vector<lumx_world*> init;
+    ServedWorld* last = nullptr;
     for (int wi : f.worlds) {
         ServedWorld& sw = s->worlds[wi];
-        if (sw.ready_pending) {
-            HIP_TRY(hipStreamWaitEvent(st, sw.ready, 0));
-            sw.ready_pending = false;
-        }
+        if (sw.ready_pending && (!last || sw.ready_seq > last->ready_seq)) last = &sw;
         if (sw.init_pending) init.push_back(sw.w);
     }
+    if (last) HIP_TRY(hipStreamWaitEvent(st, last->ready, 0));
+    for (int wi : f.worlds) s->worlds[wi].ready_pending = false;
     if (init.empty()) return LUMX_OK;
     // One set-up launch for all of them.
     int rc = world_attach_init_many(init.data(), (int)init.size(), st);
@@ -672,6 +693,7 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
     if (!s || !w || !id) return fail(LUMX_EINVAL, "null argument");
     if (w->ctx != s->ctx) return fail(LUMX_EINVAL, "world belongs to another context");
     std::lock_guard<std::mutex> lk(s->mu);
+    s->segs_valid = false;  // the arena may change below
     for (const ServedWorld& o : s->worlds)
         if (o.w == w) return fail(LUMX_EINVAL, "world already added");
     if (s->desc.hbm_budget) {
