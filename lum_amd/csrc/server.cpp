@@ -712,6 +712,15 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
         if (rc) return rc;
         size_t off = 0;
         if (s->arena.take(w->device_bytes, &off)) {
+            // Its static state into the snapshot now (before the move, so a
+            // failure leaves the world as it was): evictions while serving
+            // then copy only the field (a first copy into a pinned snapshot
+            // has been seen to hold the host for milliseconds).
+            rc = world_snapshot_static(w);
+            if (rc) {
+                s->arena.give(off, w->device_bytes);
+                return rc;
+            }
             hipError_t e = hipMemcpy(s->arena.base + off, w->dev_mem, w->device_bytes, hipMemcpyDeviceToDevice);
             if (e != hipSuccess) {
                 s->arena.give(off, w->device_bytes);
@@ -719,11 +728,6 @@ int lumx_server_add_world(lumx_server_t s, lumx_world_t w, int* id) {
             }
             if (w->dev_own) hipFree(w->dev_mem);
             world_rebase(w, s->arena.base + off, false);
-            // Its static state into the snapshot now: evictions while serving
-            // then copy only the field (a first copy into a pinned snapshot
-            // has been seen to hold the host for milliseconds).
-            rc = world_snapshot_static(w);
-            if (rc) return rc;
         } else {
             rc = lumx_world_evict(w);
             if (rc) return rc;
