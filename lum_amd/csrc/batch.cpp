@@ -284,6 +284,19 @@ int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_mod
                           n_models[i] > 0 ? pl[i] : nullptr, &args[i], &ends[i]);
         if (rc) return rc;
     }
+    // Each map launch's workgroups, checked before blockify changes any world.
+    {
+        size_t n_launches = 0;
+        for (int i = 0; i < n; ++i) n_launches = std::max(n_launches, ends[i].size());
+        for (size_t k = 0; k < n_launches; ++k) {
+            long long g = 0;
+            for (int i = 0; i < n; ++i) {
+                if (k >= ends[i].size()) continue;
+                for (int j = k ? ends[i][k - 1] : 0; j < ends[i][k]; ++j) g += map_tiles(args[i][j]);
+            }
+            if (g > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batched map launch too large");
+        }
+    }
     // Host blockify, world by world.  When world k fails the worlds before it
     // are still committed, as a sequence of single-world maps would leave them.
     int failed_rc = LUMX_OK;
@@ -388,8 +401,7 @@ int launcher_map(Launcher* L, lumx_world* const* worlds, int n, const int* n_mod
                 ++c;
             }
         }
-        if (g > (1ll << 31) - 1) return fail(LUMX_EINVAL, "batched map launch too large");
-        begin[c] = (int)g;
+        begin[c] = (int)g;  // below 2^31: checked before blockify
         total_groups[k] = (int)g;
     }
     for (int i = 0; i < m; ++i) {
