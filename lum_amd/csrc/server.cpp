@@ -85,6 +85,9 @@ struct Flight {
 
 // Calls queued at once by the direct engine.
 constexpr int kDirectDepth = 256;
+// Launches' worth of queued requests a batched launch looks through for
+// worlds that are resident already.
+constexpr int kLookahead = 4;
 
 // First-fit ranges of the arena, coalesced on release.
 struct Arena {
@@ -480,6 +483,46 @@ static void worker_loop(lumx_server* s) {
             size_t no_room = SIZE_MAX;
             Flight f;
             long long threads[2] = {0, 0};  // radiance, glossy threads of the launch being built
+            // With a backlog (more than two launches' worth queued): first,
+            // among the next kLookahead launches' worth of requests, those
+            // whose worlds are resident now (no swap before they can run), for
+            // up to three quarters of the launch; then the rest in queue
+            // order, below, so the head of the queue moves every launch.  A
+            // world skipped here is skipped for the rest of this pass, so each
+            // world's requests keep their order.  Without a backlog requests
+            // go strictly in queue order, which keeps the latency tail short.
+            if (!s->desc.direct && s->desc.hbm_budget && s->desc.max_batch >= 4 &&
+                s->pending.size() > 2 * (size_t)s->desc.max_batch) {
+                std::vector<char> skip(nw, 0);
+                int seen = 0;
+                const int horizon = kLookahead * s->desc.max_batch;
+                const int first_pass = s->desc.max_batch - s->desc.max_batch / 4;
+                for (auto it = s->pending.begin(); it != s->pending.end() && seen < horizon &&
+                                                   (int)f.reqs.size() < first_pass;
+                     ++seen) {
+                    Req* r = *it;
+                    const int wi = r->spec.world;
+                    bool ok = !skip[wi] && !taken[wi] && s->worlds[wi].busy == 0 && resident(s->worlds[wi].w);
+                    if (ok && !f.reqs.empty())
+                        ok = s->worlds[wi].w->desc.radiance_format ==
+                             s->worlds[f.worlds[0]].w->desc.radiance_format;
+                    const int kind = r->spec.kind == LUMX_REQ_RADIANCE ? 0 : 1;
+                    const long long t = request_threads(r->spec, s->worlds[wi].w);
+                    if (ok && threads[kind] + t > kLaunchThreads) ok = false;
+                    if (!ok) {
+                        skip[wi] = 1;
+                        ++it;
+                        continue;
+                    }
+                    taken[wi] = 1;
+                    threads[kind] += t;
+                    s->worlds[wi].busy++;
+                    s->worlds[wi].last_use = ++s->use_clock;
+                    f.reqs.push_back(r);
+                    f.worlds.push_back(wi);
+                    it = s->pending.erase(it);
+                }
+            }
             for (auto it = s->pending.begin(); it != s->pending.end() && (int)f.reqs.size() < s->desc.max_batch;) {
                 Req* r = *it;
                 const int wi = r->spec.world;
