@@ -370,7 +370,7 @@ int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
  * worlds (and evicting the idle ones the pending requests need last) to stay
  * within the HBM budget.  With a budget and more than two launches' worth
  * queued, a launch first takes up to three quarters of its requests from
- * worlds already resident among the next eight launches' worth of the queue,
+ * worlds already resident among the next sixteen launches' worth of the queue,
  * then the rest in arrival order (each world's requests still in submission
  * order).  It packs up to max_batch requests into one batched radiance launch
  * plus one batched glossy launch, and keeps up to `streams` launches in

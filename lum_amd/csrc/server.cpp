@@ -87,7 +87,7 @@ struct Flight {
 constexpr int kDirectDepth = 256;
 // Launches' worth of queued requests a batched launch looks through for
 // worlds that are resident already.
-constexpr int kLookahead = 8;
+constexpr int kLookahead = 16;
 
 // First-fit ranges of the arena, coalesced on release.
 struct Arena {
