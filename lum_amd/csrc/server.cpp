@@ -483,6 +483,16 @@ static void worker_loop(lumx_server* s) {
             size_t no_room = SIZE_MAX;
             Flight f;
             long long threads[2] = {0, 0};  // radiance, glossy threads of the launch being built
+            // Request *it of world wi joins the launch; the next one in the queue.
+            auto admit = [&](std::deque<Req*>::iterator it, int wi, int kind, long long t) {
+                taken[wi] = 1;
+                threads[kind] += t;
+                s->worlds[wi].busy++;
+                s->worlds[wi].last_use = ++s->use_clock;
+                f.reqs.push_back(*it);
+                f.worlds.push_back(wi);
+                return s->pending.erase(it);
+            };
             // With a backlog (more than two launches' worth queued): first,
             // among the next kLookahead launches' worth of requests, those
             // whose worlds are resident now (no swap before they can run), for
@@ -514,13 +524,7 @@ static void worker_loop(lumx_server* s) {
                         ++it;
                         continue;
                     }
-                    taken[wi] = 1;
-                    threads[kind] += t;
-                    s->worlds[wi].busy++;
-                    s->worlds[wi].last_use = ++s->use_clock;
-                    f.reqs.push_back(r);
-                    f.worlds.push_back(wi);
-                    it = s->pending.erase(it);
+                    it = admit(it, wi, kind, t);
                 }
             }
             for (auto it = s->pending.begin(); it != s->pending.end() && (int)f.reqs.size() < s->desc.max_batch;) {
@@ -531,11 +535,7 @@ static void worker_loop(lumx_server* s) {
                     // are ordered by the stream, so it need not be idle.
                     if (!f.reqs.empty()) break;
                     if (!resident(s->worlds[wi].w) && !ensure_resident(s, wi, taken, next_use(s))) break;
-                    s->worlds[wi].busy++;
-                    s->worlds[wi].last_use = ++s->use_clock;
-                    f.reqs.push_back(r);
-                    f.worlds.push_back(wi);
-                    it = s->pending.erase(it);
+                    admit(it, wi, r->spec.kind == LUMX_REQ_RADIANCE ? 0 : 1, 0);
                     break;
                 }
                 bool ok = !blocked[wi] && !taken[wi] && s->worlds[wi].busy == 0;
@@ -565,13 +565,7 @@ static void worker_loop(lumx_server* s) {
                     ++it;
                     continue;
                 }
-                taken[wi] = 1;
-                threads[kind] += t;
-                s->worlds[wi].busy++;
-                s->worlds[wi].last_use = ++s->use_clock;
-                f.reqs.push_back(r);
-                f.worlds.push_back(wi);
-                it = s->pending.erase(it);
+                it = admit(it, wi, kind, t);
             }
             if (!f.reqs.empty()) {
                 int rc = LUMX_OK;
