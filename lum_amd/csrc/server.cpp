@@ -9,7 +9,10 @@
 //      of the same world was passed over, and the world is (or can be made)
 //      resident within the HBM budget by evicting idle worlds (those the
 //      pending queue names last, then the cheapest, then the least recently
-//      used);
+//      used).  With a budget and a backlog, up to three quarters of the
+//      launch first come from requests whose worlds are resident already,
+//      within the next kLookahead launches' worth of the queue (each world's
+//      requests still in order);
 //   3. the taken radiance requests go into one batched radiance launch and
 //      the glossy requests into one batched glossy launch, on the next stream,
 //      followed by a completion event.
@@ -21,9 +24,12 @@
 // out first-fit with coalescing) or as a pinned host snapshot; worlds added
 // while the arena has room move into it.  Evictions (device -> snapshot)
 // and restores (snapshot -> arena) are copies queued on the server's copy
-// stream, in stream order with each other; a launch that needs a restored
-// world waits for the copy stream with an event, so the worker never blocks
-// on a swap and no allocation happens while serving.
+// stream, in stream order with each other; a launch that needs restored
+// worlds waits for the latest of those restores with one event and sets up
+// the rest of their blocks (one batched kernel) on its own stream, so the
+// copy stream carries copies only, the worker never blocks on a swap and no
+// allocation happens while serving.  While every launch slot is busy the
+// worker restores the worlds of the next requests ahead of time (prefetch).
 //
 // Direct mode (the plain engine): each request is its own lumx_radiance_update
 // or lumx_glossy call, with kernel arguments by value, queued on one stream in
