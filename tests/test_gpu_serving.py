@@ -245,3 +245,59 @@ def test_submit_rejects_what_a_launch_would(gpu_ctx):
     finally:
         srv.close()
         w.close()
+
+
+def test_launch_restoring_more_than_sixteen_worlds(gpu_ctx):
+    """40 evicted worlds, one request each, 32 per launch: the first launch
+    restores 32 worlds at once (their blocks are set up in launches of 16);
+    every world ends as its update alone leaves it."""
+    import torch
+    from lum_amd.api import World
+    n, size = 40, (8, 8, 8)
+
+    def build(i):
+        rng = np.random.default_rng(500 + i)
+        blocks = synth.make_world(size, rng, 32)
+        pal = synth.make_palette(rng, 32)
+        mats = synth.make_materials(rng)
+        field = synth.make_field(size, rng)
+        w = World(gpu_ctx, size, static_blocks=32, palette_slots=32, radiance_format=RADIANCE_F16)
+        w.set_blocks(blocks)
+        w.set_palette(pal)
+        w.set_materials(mats)
+        w.write_radiance(field)
+        w.upload_blocks()
+        return w
+
+    rng = np.random.default_rng(77)
+    reqs = []
+    for i in range(n):
+        k = int(rng.integers(1, 513))
+        flat = rng.permutation(512)[:k]
+        req = np.stack([flat % 8, (flat // 8) % 8, flat // 64, np.zeros(k, int)], 1).astype(np.int32)
+        reqs.append((radiance_params(rays=64, time_seed=i, alpha=0.2, max_steps=64), torch.from_numpy(req).cuda()))
+    alone = []
+    for i in range(n):
+        w = build(i)
+        w.radiance_update(reqs[i][1], reqs[i][0])
+        torch.cuda.synchronize()
+        alone.append(w.read_radiance())
+        w.close()
+    worlds = [build(i) for i in range(n)]
+    per_world = worlds[0].resident()[1]
+    torch.cuda.synchronize()
+    for w in worlds:
+        w.evict()
+    srv = Server(gpu_ctx, worlds, hbm_budget=n * per_world, max_batch=32, streams=2)
+    tickets = [srv.submit(radiance_request(srv.ids[i], reqs[i][0], reqs[i][1])) for i in range(n)]
+    for t in tickets:
+        srv.wait(t)
+    srv.drain()
+    stats = srv.stats()
+    srv.close()
+    assert stats["completed"] == n and stats["restores"] == n and stats["evictions"] == 0
+    assert stats["launches"] < n
+    for i, w in enumerate(worlds):
+        w.restore()
+        assert np.array_equal(w.read_radiance(), alone[i]), i
+        w.close()
