@@ -368,8 +368,8 @@ int lumx_batch_advance_frame(lumx_batch_t b, void* stream);
  * A worker thread admits submitted requests in arrival order, at most one
  * per world per launch and in submission order per world, restoring evicted
  * worlds (and evicting the idle ones the pending requests need last) to stay
- * within the HBM budget.  With a budget and more than two launches' worth
- * queued, a launch first takes up to three quarters of its requests from
+ * within the HBM budget.  With a budget, max_batch >= 4 and more than two
+ * launches' worth queued, a launch first takes up to three quarters of its requests from
  * worlds already resident among the next sixteen launches' worth of the queue,
  * then the rest in arrival order (each world's requests still in submission
  * order).  It packs up to max_batch requests into one batched radiance launch
