@@ -181,7 +181,9 @@ int lumx_world_radiance_dev(lumx_world_t w, void** ptr, size_t* bytes);
 int lumx_world_upload_blocks(lumx_world_t w, void* stream);
 /* Residency: evict copies the world's device state (blockified world,
  * palette slots in use, materials, current radiance field) to host memory and
- * frees its HBM; restore re-creates it bit for bit.  Passes on an evicted
+ * frees its HBM; restore re-creates it bit for bit (in a new device block,
+ * the current field possibly in the other of its two buffers), so graphs
+ * recorded before an eviction must be recorded again.  Passes on an evicted
  * world fail with LUMX_EINVAL.  No reference counterpart (the reference keeps
  * one world resident); used by the serving engine's HBM budget. */
 int lumx_world_evict(lumx_world_t w);
