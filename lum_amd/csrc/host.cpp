@@ -363,10 +363,6 @@ void lumx::world_rebase(lumx_world* w, void* block, bool own) {
 }
 }  // extern "C++"
 
-// Idle values of everything a world does not keep while evicted: the
-// other field and the palette slots above `used` zeroed, claims idle,
-// counters zero (the map counter `counter`), occupancy bits rebuilt from the
-// blocks and the palette (one kernel on s).
 // What world_attach_kernel needs to set up the rest of w's block, palette
 // slots below `used` kept (the next map re-uploads the origin world).
 static lumx::AttachArgs attach_args(lumx_world* w, int used) {
@@ -390,6 +386,10 @@ static lumx::AttachArgs attach_args(lumx_world* w, int used) {
     return a;
 }
 
+// Idle values of everything a world does not keep while evicted: the
+// other field and the palette slots above `used` zeroed, claims idle,
+// counters zero (the map counter `counter`), occupancy bits rebuilt from the
+// blocks and the palette (one kernel on s).
 static hipError_t world_init_rest(lumx_world* w, int used, hipStream_t s) {
     return lumx::launch_world_attach(attach_args(w, used), s);
 }
