@@ -140,13 +140,13 @@ struct Arena {
     }
 };
 
-}  // namespace
-
 // A run of the arena in address order: a free gap (k = -1) or resident world k.
 struct ArenaSeg {
     size_t off, size;
     int k;
 };
+
+}  // namespace
 
 struct lumx_server {
     lumx_context* ctx = nullptr;
